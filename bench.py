@@ -1,0 +1,209 @@
+#!/usr/bin/env python3
+"""Benchmark: sphere BVE direct-sum RK4 steps on one MI355X.
+
+Workload (N = 655k): icosahedral mesh at level 8, 655,362 vertex particles plus
+1,310,720 panel-centre particles (T = 1,966,082 targets, S = 1,310,720 active
+sources), relative vorticity = 4 pi exp(-16 |x - x0|^2) (x0 at lat 0.3, lon 0)
+plus a Rossby-Haurwitz-4 background, Omega = 2 pi, dt = 0.01.  One step is a
+classical RK4 step on the device (4 Biot-Savart sums of T*S - S pairs, FP32
+pair arithmetic with compensated accumulation, FP64 stage algebra, stage
+points projected onto the sphere in FP64).
+
+Prints one JSON line.  `value` = G pair-interactions/s over the timed steps.
+Extra fields: RK4 steps/s, velocity error of the FP32 sum against the FP64 GPU
+sum on a block of targets, the pair kernel's roofline, and the CPU baseline
+(the FP64 oracle on the host cores over a bounded block of targets).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "lpm-v2_amd"))
+sys.path.insert(0, ROOT)
+
+METRIC = "G pair-interactions/s & RK4 steps/s at N=655k; speedup vs plain at stated err"
+FLOPS_PER_PAIR = 23  # sphere Biot-Savart flop contract
+PEAK_FP32_TFLOPS = 157.3
+PEAK_HBM_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def build_workload(level: int):
+    from lpmgpu import ic, mesh
+
+    m = mesh.icosahedral_mesh(level)
+    x0 = ic.lat_lon_point(0.3, 0.0)
+    zeta = ic.gaussian_vortex(m.x, m.y, m.z, x0, 4.0 * np.pi, 16.0) + ic.rh4_vorticity(m.x, m.y, m.z)
+    return m, zeta
+
+
+def cpu_baseline(m, zeta, target_seconds: float, max_threads: int):
+    """FP64 oracle (the reference loop restated in C, OpenMP over a contiguous
+    target block) on a bounded block of targets; returns G pairs/s."""
+    from oracle import oracle
+
+    threads = oracle.set_threads(max_threads)
+    ns = int(m.active.sum())
+    b = 256
+    rate = None
+    t_total = 0.0
+    while True:
+        t0 = time.perf_counter()
+        oracle.bve_sphere_velocity(m.x, m.y, m.z, zeta, m.area, m.active, 1.0, 0, b)
+        dt = time.perf_counter() - t0
+        t_total += dt
+        pairs = b * ns - int(m.active[:b].sum())
+        rate = pairs / dt
+        if dt > 0.25 * target_seconds or t_total > target_seconds or b >= m.n:
+            break
+        b = min(m.n, int(b * max(2.0, 0.5 * target_seconds / max(dt, 1e-3))))
+    return {
+        "value": rate / 1e9,
+        "unit": "G pair-interactions/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": f"FP64 oracle BVESphereVelocity loop, targets [0,{b}) x {ns} active sources "
+                  f"({pairs / 1e9:.2f} G pairs, {dt:.1f} s)",
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--level", type=int, default=8)
+    ap.add_argument("--dt", type=float, default=0.01)
+    ap.add_argument("--err-block", type=int, default=32768, help="targets checked against FP64 (0 = skip)")
+    ap.add_argument("--cpu-seconds", type=float, default=10.0, help="CPU baseline budget (0 = skip)")
+    ap.add_argument("--cpu-threads", type=int, default=int(os.environ.get("OMP_NUM_THREADS", "16")))
+    ap.add_argument("--graph", action="store_true", help="replay steps as hipGraphs (no per-kernel timing)")
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build measures one GPU; the direct sum shards by target blocks "
+                          "(see DESIGN.md), --gpus N > 1 is not run"}))
+        sys.exit(2)
+
+    import lpmgpu
+    from lpmgpu import _lib as L
+
+    t0 = time.perf_counter()
+    m, zeta = build_workload(args.level)
+    T, S = m.n, int(m.active.sum())
+    pairs_per_sum = T * S - S
+    log(f"[bench] mesh level {args.level}: T={T} S={S} built in {time.perf_counter() - t0:.1f}s")
+
+    ctx = lpmgpu.Context(0)
+    ctx.set_timing(not args.graph)
+    omega = 2.0 * np.pi
+    solver = lpmgpu.BVESolver(ctx, m.x, m.y, m.z, zeta, m.area, m.active, radius=1.0, rotation_rate=omega,
+                              precision="fp32", use_graph=args.graph, project=True)
+    log("[bench] solver ready (initial velocity computed)")
+
+    # accuracy of the FP32 sum vs the FP64 GPU sum on a block of targets
+    err = {}
+    if args.err_block > 0:
+        b = min(args.err_block, T)
+        i0 = m.n_vertices - b // 2  # straddle vertices and centres
+        i1 = i0 + b
+        g32 = np.array(ctx.bve_sphere_velocity(m.x, m.y, m.z, zeta, m.area, m.active, i0=i0, i1=i1,
+                                               precision="fp32"))[:, i0:i1]
+        g64 = np.array(ctx.bve_sphere_velocity(m.x, m.y, m.z, zeta, m.area, m.active, i0=i0, i1=i1,
+                                               precision="fp64"))[:, i0:i1]
+        d = np.sqrt(((g32 - g64) ** 2).sum(0))
+        mag = np.sqrt((g64**2).sum(0))
+        err = {
+            "max_rel": float(d.max() / mag.max()),
+            "rms_rel": float(np.sqrt((d**2).sum() / (mag**2).sum())),
+            "pointwise_max_rel": float((d / np.maximum(mag, 1e-3 * mag.max())).max()),
+            "targets": [int(i0), int(i1)],
+            "vs": "FP64 GPU direct sum",
+        }
+        log(f"[bench] FP32 vs FP64 on {b} targets: {err}")
+
+    for _ in range(args.warmup):
+        solver.timestep(args.dt)
+    ctx.synchronize()
+    ctx.reset_kernel_stats()
+    log(f"[bench] warmup {args.warmup} done")
+
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        solver.timestep(args.dt)
+        log(f"[bench] step {k + 1}/{args.steps} done")
+    ctx.synchronize()
+    elapsed = time.perf_counter() - t0
+    ms_per_step = 1e3 * elapsed / args.steps
+    total_pairs = 4.0 * pairs_per_sum * args.steps
+    value = total_pairs / elapsed / 1e9
+
+    launches, kms = ctx.kernel_stats(L.K_SPHERE_BS)
+    roofline = None
+    if launches > 0:
+        avg_s = kms / launches / 1e3
+        achieved = FLOPS_PER_PAIR * pairs_per_sum / avg_s / 1e12
+        roofline = {
+            "bound": "valu",
+            "achieved": round(achieved, 3),
+            "peak": PEAK_FP32_TFLOPS,
+            "unit": "TFLOP/s",
+            "frac": round(achieved / PEAK_FP32_TFLOPS, 4),
+            "traffic": None,
+            "kernel": "pair_sum_kernel<SphereBS<float>>",
+            "avg_launch_ms": round(avg_s * 1e3, 3),
+            "launches": launches,
+            "flops_per_launch": FLOPS_PER_PAIR * pairs_per_sum,
+        }
+    ke, ens, circ = solver.diagnostics()
+    solver.delete()
+
+    cpu = None
+    if args.cpu_seconds > 0:
+        log("[bench] CPU baseline ...")
+        cpu = cpu_baseline(m, zeta, args.cpu_seconds, args.cpu_threads)
+        log(f"[bench] CPU baseline {cpu}")
+
+    out = {
+        "metric": METRIC,
+        "value": round(value, 3),
+        "unit": "G pair-interactions/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32",
+        "data": "synthetic",
+        "config": {
+            "workload": f"sphere BVE plain direct-sum RK4, icosahedral level {args.level} "
+                        f"(T={T}, S={S}), Gaussian vortex + RH4, FP32-compensated pairs, FP64 RK4",
+            "targets": T,
+            "sources": S,
+            "pairs_per_step": 4 * pairs_per_sum,
+            "dt": args.dt,
+        },
+        "rk4_steps_per_s": round(1e3 / ms_per_step, 4),
+        "velocity_error": err,
+        "diagnostics": {"ke": ke, "enstrophy": ens, "circulation": circ},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+    ctx.close()
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,202 @@
+/*
+ * lpm_direct.h -- C ABI of the MI355X (gfx950) particle direct-sum library.
+ *
+ * Drop-in boundary for lpm-v2's O(T*S) direct sums.  Every entry point takes
+ * plain pointers and sizes (ISO_C_BINDING compatible: double = real(c_double),
+ * int = integer(c_int), int32_t = integer(c_int32_t)), returns an int status
+ * (LPM_OK = 0) and never aborts; on failure lpm_last_error() gives the reason.
+ *
+ * Conventions shared by all entry points
+ *  - arrays are 0-based SoA of length n; the target block is the half-open
+ *    range [i0, i1).  A Fortran MPISetup block maps as i0 = indexStart - 1,
+ *    i1 = indexEnd (MPISetup.f90:36-42, 132-146);
+ *  - active[j] != 0 marks an active particle (the reference's
+ *    logical(klog) isActive / activeMask; any nonzero value is true);
+ *  - outputs are written for targets i0..i1-1 only, other entries untouched;
+ *  - host-pointer calls are synchronous: inputs are copied to HBM, the sum
+ *    runs, outputs are copied back before the call returns;
+ *  - precision: LPM_PREC_FP64 runs the pair kernel in FP64 (the plain
+ *    drop-in), LPM_PREC_FP32_KAHAN runs it in FP32 with compensated
+ *    per-target accumulation and FP64 set-up/epilogue.
+ * A context is not thread-safe; use one per host thread / MPI rank.
+ */
+#ifndef LPM_DIRECT_H
+#define LPM_DIRECT_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* status codes */
+#define LPM_OK 0
+#define LPM_ERR_ARG 1         /* invalid argument (null pointer, bad range, ...) */
+#define LPM_ERR_HIP 2         /* HIP runtime error                                */
+#define LPM_ERR_NOMEM 3       /* device allocation failed                         */
+#define LPM_ERR_STATE 4       /* object in the wrong state for the call           */
+#define LPM_ERR_UNSUPPORTED 5 /* option not supported                             */
+
+/* precision modes */
+#define LPM_PREC_FP64 0
+#define LPM_PREC_FP32_KAHAN 1
+
+/* pair-kernel kinds (timing statistics and tuning) */
+#define LPM_K_SPHERE_BS 0    /* sphere Biot-Savart velocity     */
+#define LPM_K_PLANE_BS 1     /* planar Biot-Savart velocity     */
+#define LPM_K_PSE8 2         /* PSE Laplacian, 8th-order kernel */
+#define LPM_K_PSE2 3         /* PSE Laplacian, 2nd-order kernel */
+#define LPM_K_SPHERE_GREEN 4 /* sphere stream functions         */
+#define LPM_K_PLANE_GREEN 5  /* planar stream function          */
+#define LPM_K_COUNT 6
+
+typedef struct lpm_ctx lpm_ctx;
+
+/* Library version as 10000*major + 100*minor + patch. */
+int lpm_version(void);
+
+/* Context on HIP device `device` with its own stream and workspace.  On
+ * failure *ctx is NULL and lpm_last_error(NULL) gives the reason. */
+int lpm_create(lpm_ctx **ctx, int device);
+int lpm_destroy(lpm_ctx *ctx);
+/* Message of the last failed call on ctx (or of lpm_create when ctx is NULL);
+ * "" when the last call succeeded. */
+const char *lpm_last_error(const lpm_ctx *ctx);
+int lpm_synchronize(lpm_ctx *ctx);
+
+/* ---------------------------------------------------------------- sums -- */
+
+/* Sphere Biot-Savart velocity.  Replaces BVESphereVelocity
+ * (SphereBVESolver.f90:377-430; same arguments in the same order plus the
+ * explicit block) and the public duplicate setVelocityFromVorticity
+ * (SphereBVE.f90:489-531).
+ *   u_i = sum_{active j != i} (x_i x x_j) (-zeta_j A_j) / (4 pi R (R^2 - x_i.x_j))
+ * rotation_rate is accepted and unused, as in the reference. */
+int lpm_bve_sphere_velocity(lpm_ctx *ctx, double *u, double *v, double *w, const double *x,
+                            const double *y, const double *z, const double *relvort,
+                            const double *area, double radius, double rotation_rate,
+                            const int32_t *active, int n, int i0, int i1, int precision);
+
+/* Planar Biot-Savart velocity (singular point vortices).  Replaces
+ * planarIncompressibleVelocity (PlaneIncompressibleSolver.f90:278-316) and
+ * PlanarIncompressible.f90:426-466.
+ *   u_i = -sum (y_i - y_j) s_ij,  v_i = sum (x_i - x_j) s_ij,
+ *   s_ij = omega_j A_j / (2 pi |x_i - x_j|^2),  active j != i */
+int lpm_plane_velocity(lpm_ctx *ctx, double *u, double *v, const double *x, const double *y,
+                       const double *vort, const double *area, const int32_t *active, int n,
+                       int i0, int i1, int precision);
+
+/* PSE Laplacian on the plane.  Replaces PSEPlaneLaplacianAtParticles
+ * (PSEDirectSum.f90:467-500), which takes eps from PSE%eps (PSEDirectSum.f90:56-60).
+ *   lap_i = eps^-4 sum_{active j} (f_j - f_i) K(|x_i - x_j| / eps) A_j
+ * order 8: K = bivariateLaplacianKernel8 (PSEDirectSum.f90:622-627);
+ * order 2: K = (4/pi) exp(-rho^2). */
+int lpm_pse_plane_laplacian(lpm_ctx *ctx, double *lap, const double *x, const double *y,
+                            const double *f, const double *area, const int32_t *active, int n,
+                            int i0, int i1, double eps, int order, int precision);
+
+/* Sphere stream functions.  Replaces SetStreamFunctionsOnMesh (SphereBVE.f90:445-485).
+ *   psi_i = sum_{active j != i} -log(R^2 - x_i.x_j) / (4 pi) zeta_j A_j
+ * for zeta = relvort (psi_rel) and zeta = absvort (psi_abs). */
+int lpm_bve_sphere_stream(lpm_ctx *ctx, double *psi_rel, double *psi_abs, const double *x,
+                          const double *y, const double *z, const double *relvort,
+                          const double *absvort, const double *area, double radius,
+                          const int32_t *active, int n, int i0, int i1, int precision);
+
+/* Planar stream function.  Replaces SetStreamFunctionOnMesh (PlanarIncompressible.f90:470-505).
+ *   psi_i = sum_{active j != i} log(|x_i - x_j|) / (2 pi) omega_j A_j */
+int lpm_plane_stream(lpm_ctx *ctx, double *psi, const double *x, const double *y,
+                     const double *vort, const double *area, const int32_t *active, int n,
+                     int i0, int i1, int precision);
+
+/* ------------------------------------------------------------ helpers -- */
+
+/* PSE%eps = max_edge_length ** power (PSEDirectSum.f90:93-113, default power 0.75). */
+int lpm_pse_eps(double max_edge_length, double power, double *eps);
+
+/* MPISetup LoadBalance (MPISetup.f90:132-146): 1-based inclusive blocks. */
+int lpm_load_balance(int n_items, int n_procs, int *index_start, int *index_end,
+                     int *message_length);
+
+/* FP64 conserved integrals over active particles: TotalKE and TotalEnstrophy
+ * (SphereBVE.f90:410-441; PlanarIncompressible.f90:368-401) and the total
+ * circulation sum zeta A.  w may be NULL (plane). */
+int lpm_diagnostics(lpm_ctx *ctx, const double *u, const double *v, const double *w,
+                    const double *vort, const double *area, const int32_t *active, int n,
+                    double *ke, double *enstrophy, double *circulation);
+
+/* ------------------------------------------- device-resident RK4 solvers -- */
+
+/* Sphere BVE solver: the state (positions, relative vorticity, stored
+ * velocity) lives in HBM for the solver's lifetime.  Mirrors BVESolver
+ * New / Timestep / Delete (SphereBVESolver.f90:80-90, 112-353).
+ * u, v, w may be NULL: the velocity is then computed from the state (the
+ * SetVelocityOnMesh call drivers make before New).  absvort may be NULL when
+ * no stream functions are wanted. */
+typedef struct lpm_bve_solver lpm_bve_solver;
+
+#define LPM_SOLVER_STREAM_EACH_STEP 1 /* end every step with the stream functions (SphereBVESolver.f90:352) */
+#define LPM_SOLVER_USE_GRAPH 2        /* replay each RK4 step as a captured hipGraph */
+/* Project every RK stage input and every new position back onto the sphere
+ * of radius R in FP64 (x <- R x / |x|) before the velocity sum.  The
+ * reference does not (SphereBVESolver.f90:249-329): its R^2 - x.y turns
+ * negative for near pairs once the stage displacement exceeds the particle
+ * spacing, so the un-projected scheme is only usable at coarse resolution. */
+#define LPM_SOLVER_PROJECT_STAGES 4
+
+int lpm_bve_solver_new(lpm_ctx *ctx, lpm_bve_solver **solver, int n, const double *x,
+                       const double *y, const double *z, const double *relvort,
+                       const double *absvort, const double *area, const int32_t *active,
+                       const double *u, const double *v, const double *w, double radius,
+                       double rotation_rate, int precision, int flags);
+/* One classical RK4 step (timestepPrivate, SphereBVESolver.f90:219-353). */
+int lpm_bve_solver_timestep(lpm_bve_solver *solver, double dt);
+int lpm_bve_solver_steps(lpm_bve_solver *solver, double dt, int nsteps);
+/* Copies the state back; any pointer may be NULL. */
+int lpm_bve_solver_get_state(lpm_bve_solver *solver, double *x, double *y, double *z,
+                             double *relvort, double *u, double *v, double *w);
+/* Recomputes the stream functions of the current state into host arrays. */
+int lpm_bve_solver_stream(lpm_bve_solver *solver, double *psi_rel, double *psi_abs);
+int lpm_bve_solver_diagnostics(lpm_bve_solver *solver, double *ke, double *enstrophy,
+                               double *circulation);
+int lpm_bve_solver_delete(lpm_bve_solver *solver);
+
+/* Planar incompressible solver: PlaneSolver New / Timestep / Delete
+ * (PlaneIncompressibleSolver.f90:66-76, 103-259).  Vorticity is carried. */
+typedef struct lpm_plane_solver lpm_plane_solver;
+
+int lpm_plane_solver_new(lpm_ctx *ctx, lpm_plane_solver **solver, int n, const double *x,
+                         const double *y, const double *vort, const double *area,
+                         const int32_t *active, const double *u, const double *v,
+                         int precision, int flags);
+int lpm_plane_solver_timestep(lpm_plane_solver *solver, double dt);
+int lpm_plane_solver_steps(lpm_plane_solver *solver, double dt, int nsteps);
+int lpm_plane_solver_get_state(lpm_plane_solver *solver, double *x, double *y, double *u,
+                               double *v);
+int lpm_plane_solver_stream(lpm_plane_solver *solver, double *psi);
+int lpm_plane_solver_diagnostics(lpm_plane_solver *solver, double *ke, double *enstrophy,
+                                 double *circulation);
+int lpm_plane_solver_delete(lpm_plane_solver *solver);
+
+/* ----------------------------------------------------- timing / tuning -- */
+
+/* When enabled, every pair-kernel launch is bracketed by HIP events on the
+ * context's stream; lpm_kernel_stats returns launches and summed milliseconds
+ * per kind since the last reset. */
+int lpm_set_timing(lpm_ctx *ctx, int enable);
+int lpm_kernel_stats(lpm_ctx *ctx, int kind, int64_t *launches, double *total_ms);
+int lpm_reset_kernel_stats(lpm_ctx *ctx);
+
+/* Launch shape of a pair kernel: targets per lane (1, 2, 4 or 8), sources
+ * per compensated tile (multiple of 8) and source splits (0 = automatic).
+ * Values 0 restore the defaults. */
+int lpm_set_tuning(lpm_ctx *ctx, int kind, int precision, int targets_per_lane,
+                   int tile_sources, int splits);
+int lpm_get_tuning(lpm_ctx *ctx, int kind, int precision, int *targets_per_lane,
+                   int *tile_sources, int *splits);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* LPM_DIRECT_H */

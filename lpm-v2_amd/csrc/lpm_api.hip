@@ -1,0 +1,420 @@
+// lpm_api.hip -- C ABI: contexts, host-pointer drop-in sums, helpers,
+// timing and tuning.  Device-resident solvers live in lpm_solver.hip.
+#include <string.h>
+
+#include <string>
+
+#include "lpm_internal.hpp"
+
+using lpm::DevBuf;
+using lpm::fail;
+using lpm::hip_fail;
+
+namespace {
+
+thread_local std::string g_create_error;
+
+int check_block(lpm_ctx* ctx, int n, int i0, int i1) {
+  if (n < 0) return fail(ctx, LPM_ERR_ARG, "n must be >= 0");
+  if (i0 < 0 || i1 < i0 || i1 > n)
+    return fail(ctx, LPM_ERR_ARG,
+                "target block [i0, i1) must satisfy 0 <= i0 <= i1 <= n (got i0=" + std::to_string(i0) +
+                    ", i1=" + std::to_string(i1) + ", n=" + std::to_string(n) + ")");
+  return LPM_OK;
+}
+
+int check_prec(lpm_ctx* ctx, int precision) {
+  if (precision != LPM_PREC_FP64 && precision != LPM_PREC_FP32_KAHAN)
+    return fail(ctx, LPM_ERR_ARG, "precision must be LPM_PREC_FP64 (0) or LPM_PREC_FP32_KAHAN (1)");
+  return LPM_OK;
+}
+
+// Upload k host arrays of n doubles into ctx->in[0..k).
+int upload(lpm_ctx* ctx, int n, std::initializer_list<const double*> arrays) {
+  int k = 0;
+  for (const double* a : arrays) {
+    DevBuf& b = ctx->in[k++];
+    if (!a) continue;
+    LPM_HIP(ctx, b.ensure((size_t)std::max(n, 1) * sizeof(double)));
+    if (n > 0)
+      LPM_HIP(ctx, hipMemcpyAsync(b.p, a, (size_t)n * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+  }
+  return LPM_OK;
+}
+
+int prepare_outputs(lpm_ctx* ctx, int n, int nout) {
+  for (int k = 0; k < nout; ++k) LPM_HIP(ctx, ctx->out[k].ensure((size_t)std::max(n, 1) * sizeof(double)));
+  return LPM_OK;
+}
+
+int download(lpm_ctx* ctx, int i0, int i1, std::initializer_list<double*> outs) {
+  int k = 0;
+  for (double* o : outs) {
+    DevBuf& b = ctx->out[k++];
+    if (!o || i1 <= i0) continue;
+    LPM_HIP(ctx, hipMemcpyAsync(o + i0, b.as<double>() + i0, (size_t)(i1 - i0) * sizeof(double),
+                                hipMemcpyDeviceToHost, ctx->stream));
+  }
+  LPM_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return lpm::collect_timing(ctx);
+}
+
+}  // namespace
+
+extern "C" {
+
+int lpm_version(void) { return LPM_VERSION_NUMBER; }
+
+int lpm_create(lpm_ctx** out, int device) {
+  if (!out) {
+    g_create_error = "lpm_create: ctx pointer is NULL";
+    return LPM_ERR_ARG;
+  }
+  *out = nullptr;
+  int ndev = 0;
+  hipError_t e = hipGetDeviceCount(&ndev);
+  if (e != hipSuccess || ndev <= 0) {
+    g_create_error = std::string("lpm_create: no HIP device available (") +
+                     (e != hipSuccess ? hipGetErrorString(e) : "0 devices") + ")";
+    return LPM_ERR_HIP;
+  }
+  if (device < 0 || device >= ndev) {
+    g_create_error = "lpm_create: device " + std::to_string(device) + " out of range (" +
+                     std::to_string(ndev) + " devices)";
+    return LPM_ERR_ARG;
+  }
+  e = hipSetDevice(device);
+  if (e != hipSuccess) {
+    g_create_error = std::string("lpm_create: hipSetDevice failed: ") + hipGetErrorString(e);
+    return LPM_ERR_HIP;
+  }
+  lpm_ctx* ctx = new lpm_ctx();
+  ctx->device = device;
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, device) == hipSuccess && prop.multiProcessorCount > 0)
+    ctx->num_cus = prop.multiProcessorCount;
+  e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
+  if (e != hipSuccess) {
+    g_create_error = std::string("lpm_create: hipStreamCreate failed: ") + hipGetErrorString(e);
+    delete ctx;
+    return LPM_ERR_HIP;
+  }
+  g_create_error.clear();
+  *out = ctx;
+  return LPM_OK;
+}
+
+int lpm_destroy(lpm_ctx* ctx) {
+  if (!ctx) return LPM_ERR_ARG;
+  (void)hipSetDevice(ctx->device);
+  if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+  for (auto& b : ctx->in) b.release();
+  for (auto& b : ctx->out) b.release();
+  ctx->list.release();
+  ctx->tgt.release();
+  ctx->src.release();
+  ctx->partial.release();
+  ctx->red.release();
+  for (auto& ep : ctx->pending) {
+    (void)hipEventDestroy(ep.a);
+    (void)hipEventDestroy(ep.b);
+  }
+  for (auto e : ctx->event_pool) (void)hipEventDestroy(e);
+  if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+  delete ctx;
+  return LPM_OK;
+}
+
+const char* lpm_last_error(const lpm_ctx* ctx) {
+  if (!ctx) return g_create_error.c_str();
+  return ctx->err.c_str();
+}
+
+int lpm_synchronize(lpm_ctx* ctx) {
+  if (!ctx) return LPM_ERR_ARG;
+  LPM_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return lpm::collect_timing(ctx);
+}
+
+int lpm_bve_sphere_velocity(lpm_ctx* ctx, double* u, double* v, double* w, const double* x,
+                            const double* y, const double* z, const double* relvort,
+                            const double* area, double radius, double rotation_rate,
+                            const int32_t* active, int n, int i0, int i1, int precision) {
+  (void)rotation_rate;  // unused in the reference too (SphereBVESolver.f90:388)
+  if (!ctx) return LPM_ERR_ARG;
+  ctx->err.clear();
+  if (int rc = check_block(ctx, n, i0, i1)) return rc;
+  if (int rc = check_prec(ctx, precision)) return rc;
+  if (!u || !v || !w || !x || !y || !z || !relvort || !area || !active)
+    return fail(ctx, LPM_ERR_ARG, "lpm_bve_sphere_velocity: null array argument");
+  if (!(radius > 0.0)) return fail(ctx, LPM_ERR_ARG, "lpm_bve_sphere_velocity: radius must be > 0");
+  LPM_HIP(ctx, hipSetDevice(ctx->device));
+  if (i1 == i0) return LPM_OK;
+  int ns = 0;
+  if (int rc = lpm::upload_active_list(ctx, active, n, ctx->list, nullptr, &ns)) return rc;
+  if (int rc = upload(ctx, n, {x, y, z, relvort, area})) return rc;
+  if (int rc = prepare_outputs(ctx, n, 3)) return rc;
+  lpm::PairProblem pp;
+  pp.kind = LPM_K_SPHERE_BS;
+  pp.precision = precision;
+  pp.n = n;
+  pp.i0 = i0;
+  pp.i1 = i1;
+  pp.x = ctx->in[0].as<double>();
+  pp.y = ctx->in[1].as<double>();
+  pp.z = ctx->in[2].as<double>();
+  pp.q0 = ctx->in[3].as<double>();
+  pp.area = ctx->in[4].as<double>();
+  pp.src_list = ctx->list.as<int>();
+  pp.ns = ns;
+  pp.radius = radius;
+  pp.out0 = ctx->out[0].as<double>();
+  pp.out1 = ctx->out[1].as<double>();
+  pp.out2 = ctx->out[2].as<double>();
+  if (int rc = lpm::run_pair_sum(ctx, pp)) return rc;
+  return download(ctx, i0, i1, {u, v, w});
+}
+
+int lpm_plane_velocity(lpm_ctx* ctx, double* u, double* v, const double* x, const double* y,
+                       const double* vort, const double* area, const int32_t* active, int n,
+                       int i0, int i1, int precision) {
+  if (!ctx) return LPM_ERR_ARG;
+  ctx->err.clear();
+  if (int rc = check_block(ctx, n, i0, i1)) return rc;
+  if (int rc = check_prec(ctx, precision)) return rc;
+  if (!u || !v || !x || !y || !vort || !area || !active)
+    return fail(ctx, LPM_ERR_ARG, "lpm_plane_velocity: null array argument");
+  LPM_HIP(ctx, hipSetDevice(ctx->device));
+  if (i1 == i0) return LPM_OK;
+  int ns = 0;
+  if (int rc = lpm::upload_active_list(ctx, active, n, ctx->list, nullptr, &ns)) return rc;
+  if (int rc = upload(ctx, n, {x, y, vort, area})) return rc;
+  if (int rc = prepare_outputs(ctx, n, 2)) return rc;
+  lpm::PairProblem pp;
+  pp.kind = LPM_K_PLANE_BS;
+  pp.precision = precision;
+  pp.n = n;
+  pp.i0 = i0;
+  pp.i1 = i1;
+  pp.x = ctx->in[0].as<double>();
+  pp.y = ctx->in[1].as<double>();
+  pp.q0 = ctx->in[2].as<double>();
+  pp.area = ctx->in[3].as<double>();
+  pp.src_list = ctx->list.as<int>();
+  pp.ns = ns;
+  pp.out0 = ctx->out[0].as<double>();
+  pp.out1 = ctx->out[1].as<double>();
+  if (int rc = lpm::run_pair_sum(ctx, pp)) return rc;
+  return download(ctx, i0, i1, {u, v});
+}
+
+int lpm_pse_plane_laplacian(lpm_ctx* ctx, double* lap, const double* x, const double* y,
+                            const double* f, const double* area, const int32_t* active, int n,
+                            int i0, int i1, double eps, int order, int precision) {
+  if (!ctx) return LPM_ERR_ARG;
+  ctx->err.clear();
+  if (int rc = check_block(ctx, n, i0, i1)) return rc;
+  if (int rc = check_prec(ctx, precision)) return rc;
+  if (!lap || !x || !y || !f || !area || !active)
+    return fail(ctx, LPM_ERR_ARG, "lpm_pse_plane_laplacian: null array argument");
+  if (!(eps > 0.0)) return fail(ctx, LPM_ERR_ARG, "lpm_pse_plane_laplacian: eps must be > 0");
+  if (order != 2 && order != 8)
+    return fail(ctx, LPM_ERR_ARG, "lpm_pse_plane_laplacian: order must be 2 or 8");
+  LPM_HIP(ctx, hipSetDevice(ctx->device));
+  if (i1 == i0) return LPM_OK;
+  int ns = 0;
+  if (int rc = lpm::upload_active_list(ctx, active, n, ctx->list, nullptr, &ns)) return rc;
+  if (int rc = upload(ctx, n, {x, y, f, area})) return rc;
+  if (int rc = prepare_outputs(ctx, n, 1)) return rc;
+  lpm::PairProblem pp;
+  pp.kind = order == 8 ? LPM_K_PSE8 : LPM_K_PSE2;
+  pp.precision = precision;
+  pp.n = n;
+  pp.i0 = i0;
+  pp.i1 = i1;
+  pp.x = ctx->in[0].as<double>();
+  pp.y = ctx->in[1].as<double>();
+  pp.q0 = ctx->in[2].as<double>();
+  pp.area = ctx->in[3].as<double>();
+  pp.src_list = ctx->list.as<int>();
+  pp.ns = ns;
+  pp.eps = eps;
+  pp.out0 = ctx->out[0].as<double>();
+  if (int rc = lpm::run_pair_sum(ctx, pp)) return rc;
+  return download(ctx, i0, i1, {lap});
+}
+
+int lpm_bve_sphere_stream(lpm_ctx* ctx, double* psi_rel, double* psi_abs, const double* x,
+                          const double* y, const double* z, const double* relvort,
+                          const double* absvort, const double* area, double radius,
+                          const int32_t* active, int n, int i0, int i1, int precision) {
+  if (!ctx) return LPM_ERR_ARG;
+  ctx->err.clear();
+  if (int rc = check_block(ctx, n, i0, i1)) return rc;
+  if (int rc = check_prec(ctx, precision)) return rc;
+  if (!psi_rel || !psi_abs || !x || !y || !z || !relvort || !absvort || !area || !active)
+    return fail(ctx, LPM_ERR_ARG, "lpm_bve_sphere_stream: null array argument");
+  if (!(radius > 0.0)) return fail(ctx, LPM_ERR_ARG, "lpm_bve_sphere_stream: radius must be > 0");
+  LPM_HIP(ctx, hipSetDevice(ctx->device));
+  if (i1 == i0) return LPM_OK;
+  int ns = 0;
+  if (int rc = lpm::upload_active_list(ctx, active, n, ctx->list, &ctx->in[7], &ns)) return rc;
+  if (int rc = upload(ctx, n, {x, y, z, relvort, absvort, area})) return rc;
+  if (int rc = prepare_outputs(ctx, n, 2)) return rc;
+  lpm::PairProblem pp;
+  pp.kind = LPM_K_SPHERE_GREEN;
+  pp.precision = precision;
+  pp.n = n;
+  pp.i0 = i0;
+  pp.i1 = i1;
+  pp.x = ctx->in[0].as<double>();
+  pp.y = ctx->in[1].as<double>();
+  pp.z = ctx->in[2].as<double>();
+  pp.q0 = ctx->in[3].as<double>();
+  pp.q1 = ctx->in[4].as<double>();
+  pp.area = ctx->in[5].as<double>();
+  pp.active = ctx->in[7].as<int>();
+  pp.src_list = ctx->list.as<int>();
+  pp.ns = ns;
+  pp.radius = radius;
+  pp.out0 = ctx->out[0].as<double>();
+  pp.out1 = ctx->out[1].as<double>();
+  if (int rc = lpm::run_pair_sum(ctx, pp)) return rc;
+  return download(ctx, i0, i1, {psi_rel, psi_abs});
+}
+
+int lpm_plane_stream(lpm_ctx* ctx, double* psi, const double* x, const double* y,
+                     const double* vort, const double* area, const int32_t* active, int n, int i0,
+                     int i1, int precision) {
+  if (!ctx) return LPM_ERR_ARG;
+  ctx->err.clear();
+  if (int rc = check_block(ctx, n, i0, i1)) return rc;
+  if (int rc = check_prec(ctx, precision)) return rc;
+  if (!psi || !x || !y || !vort || !area || !active)
+    return fail(ctx, LPM_ERR_ARG, "lpm_plane_stream: null array argument");
+  LPM_HIP(ctx, hipSetDevice(ctx->device));
+  if (i1 == i0) return LPM_OK;
+  int ns = 0;
+  if (int rc = lpm::upload_active_list(ctx, active, n, ctx->list, nullptr, &ns)) return rc;
+  if (int rc = upload(ctx, n, {x, y, vort, area})) return rc;
+  if (int rc = prepare_outputs(ctx, n, 1)) return rc;
+  lpm::PairProblem pp;
+  pp.kind = LPM_K_PLANE_GREEN;
+  pp.precision = precision;
+  pp.n = n;
+  pp.i0 = i0;
+  pp.i1 = i1;
+  pp.x = ctx->in[0].as<double>();
+  pp.y = ctx->in[1].as<double>();
+  pp.q0 = ctx->in[2].as<double>();
+  pp.area = ctx->in[3].as<double>();
+  pp.src_list = ctx->list.as<int>();
+  pp.ns = ns;
+  pp.out0 = ctx->out[0].as<double>();
+  if (int rc = lpm::run_pair_sum(ctx, pp)) return rc;
+  return download(ctx, i0, i1, {psi});
+}
+
+int lpm_pse_eps(double max_edge_length, double power, double* eps) {
+  if (!eps || !(max_edge_length > 0.0) || !(power > 0.0)) return LPM_ERR_ARG;
+  *eps = pow(max_edge_length, power);
+  return LPM_OK;
+}
+
+int lpm_load_balance(int n_items, int n_procs, int* index_start, int* index_end,
+                     int* message_length) {
+  if (n_items < 0 || n_procs <= 0 || !index_start || !index_end || !message_length)
+    return LPM_ERR_ARG;
+  const int chunk = n_items / n_procs;
+  for (int r = 0; r < n_procs; ++r) {
+    index_start[r] = r * chunk + 1;
+    index_end[r] = (r + 1) * chunk;
+  }
+  index_end[n_procs - 1] = n_items;
+  for (int r = 0; r < n_procs; ++r) message_length[r] = index_end[r] - index_start[r] + 1;
+  return LPM_OK;
+}
+
+int lpm_diagnostics(lpm_ctx* ctx, const double* u, const double* v, const double* w,
+                    const double* vort, const double* area, const int32_t* active, int n,
+                    double* ke, double* enstrophy, double* circulation) {
+  if (!ctx) return LPM_ERR_ARG;
+  ctx->err.clear();
+  if (n < 0) return fail(ctx, LPM_ERR_ARG, "lpm_diagnostics: n must be >= 0");
+  if (!u || !v || !vort || !area || !active || !ke || !enstrophy || !circulation)
+    return fail(ctx, LPM_ERR_ARG, "lpm_diagnostics: null argument");
+  LPM_HIP(ctx, hipSetDevice(ctx->device));
+  if (int rc = upload(ctx, n, {u, v, w, vort, area})) return rc;
+  LPM_HIP(ctx, ctx->in[7].ensure((size_t)std::max(n, 1) * sizeof(int)));
+  if (n > 0)
+    LPM_HIP(ctx, hipMemcpyAsync(ctx->in[7].p, active, (size_t)n * sizeof(int), hipMemcpyHostToDevice,
+                                ctx->stream));
+  double out[3];
+  if (int rc = lpm::run_diagnostics(ctx, n, ctx->in[0].as<double>(), ctx->in[1].as<double>(),
+                                    w ? ctx->in[2].as<double>() : nullptr, ctx->in[3].as<double>(),
+                                    ctx->in[4].as<double>(), ctx->in[7].as<int>(), out))
+    return rc;
+  *ke = out[0];
+  *enstrophy = out[1];
+  *circulation = out[2];
+  return LPM_OK;
+}
+
+int lpm_set_timing(lpm_ctx* ctx, int enable) {
+  if (!ctx) return LPM_ERR_ARG;
+  ctx->timing = enable != 0;
+  return LPM_OK;
+}
+
+int lpm_kernel_stats(lpm_ctx* ctx, int kind, int64_t* launches, double* total_ms) {
+  if (!ctx) return LPM_ERR_ARG;
+  if (kind < 0 || kind >= LPM_K_COUNT || !launches || !total_ms)
+    return fail(ctx, LPM_ERR_ARG, "lpm_kernel_stats: bad kind or null output");
+  if (int rc = lpm::collect_timing(ctx)) return rc;
+  *launches = ctx->launches[kind];
+  *total_ms = ctx->total_ms[kind];
+  return LPM_OK;
+}
+
+int lpm_reset_kernel_stats(lpm_ctx* ctx) {
+  if (!ctx) return LPM_ERR_ARG;
+  if (int rc = lpm::collect_timing(ctx)) return rc;
+  for (int k = 0; k < LPM_K_COUNT; ++k) {
+    ctx->launches[k] = 0;
+    ctx->total_ms[k] = 0.0;
+  }
+  return LPM_OK;
+}
+
+int lpm_set_tuning(lpm_ctx* ctx, int kind, int precision, int targets_per_lane, int tile_sources,
+                   int splits) {
+  if (!ctx) return LPM_ERR_ARG;
+  if (kind < 0 || kind >= LPM_K_COUNT) return fail(ctx, LPM_ERR_ARG, "lpm_set_tuning: bad kind");
+  if (int rc = check_prec(ctx, precision)) return rc;
+  if (!(targets_per_lane == 0 || targets_per_lane == 1 || targets_per_lane == 2 ||
+        targets_per_lane == 4 || targets_per_lane == 8))
+    return fail(ctx, LPM_ERR_ARG, "lpm_set_tuning: targets_per_lane must be 0, 1, 2, 4 or 8");
+  if (tile_sources < 0 || tile_sources % 8 != 0 || tile_sources > 65536)
+    return fail(ctx, LPM_ERR_ARG, "lpm_set_tuning: tile_sources must be a multiple of 8 in [0, 65536]");
+  if (splits < 0 || splits > 65536) return fail(ctx, LPM_ERR_ARG, "lpm_set_tuning: splits out of range");
+  lpm::Tuning& t = ctx->tune[kind][precision == LPM_PREC_FP32_KAHAN ? 1 : 0];
+  t.r = targets_per_lane;
+  t.kt = tile_sources;
+  t.splits = splits;
+  return LPM_OK;
+}
+
+int lpm_get_tuning(lpm_ctx* ctx, int kind, int precision, int* targets_per_lane, int* tile_sources,
+                   int* splits) {
+  if (!ctx) return LPM_ERR_ARG;
+  if (kind < 0 || kind >= LPM_K_COUNT || !targets_per_lane || !tile_sources || !splits)
+    return fail(ctx, LPM_ERR_ARG, "lpm_get_tuning: bad kind or null output");
+  if (int rc = check_prec(ctx, precision)) return rc;
+  const lpm::Tuning& t = ctx->tune[kind][precision == LPM_PREC_FP32_KAHAN ? 1 : 0];
+  *targets_per_lane = t.r;
+  *tile_sources = t.kt;
+  *splits = t.splits;
+  return LPM_OK;
+}
+
+}  // extern "C"

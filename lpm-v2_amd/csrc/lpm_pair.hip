@@ -1,0 +1,573 @@
+// lpm_pair.hip -- pack, pair-sum and finalize kernels; the pair-sum driver.
+//
+// Data flow of one sum (all on the context stream, inputs already in HBM):
+//   pack_targets / pack_sources : FP64 SoA particles -> FP32 (or FP64) records
+//   pair_sum_kernel             : all pairs, per-split FP64 partials
+//   source_totals (green only)  : FP64 sums of the source weights
+//   finalize                    : ordered sum over splits + FP64 epilogue
+//                                 (cross product, scaling, log offsets)
+#include <math.h>
+
+#include <algorithm>
+#include <map>
+
+#include "lpm_internal.hpp"
+#include "pair_kernels.hpp"
+
+namespace lpm {
+
+constexpr double kPi = 3.14159265358979323846;
+constexpr double kLn2 = 0.69314718055994530942;
+constexpr double kLog2e = 1.4426950408889634074;
+
+int fail(lpm_ctx* ctx, int code, const std::string& msg) {
+  if (ctx) ctx->err = msg;
+  return code;
+}
+
+int hip_fail(lpm_ctx* ctx, hipError_t e, const char* what) {
+  std::string m = std::string("HIP error ") + hipGetErrorName(e) + " (" + hipGetErrorString(e) +
+                  ") in " + what;
+  return fail(ctx, e == hipErrorOutOfMemory ? LPM_ERR_NOMEM : LPM_ERR_HIP, m);
+}
+
+// ------------------------------------------------------------------ packing
+struct PackArgs {
+  const double *x, *y, *z, *q0, *q1, *area;
+  const int* list;
+  int count;   // records to write
+  int i0;      // first target particle (targets)
+  double radius, sigma;
+};
+
+template <typename T>
+__device__ __forceinline__ T idx_word(int p) {
+  if constexpr (sizeof(T) == 4)
+    return __int_as_float(p);
+  else
+    return (T)p;
+}
+
+template <typename T, int KIND>
+__global__ __launch_bounds__(256) void pack_sources(PackArgs a, T* __restrict__ rec) {
+  int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= a.count) return;
+  const int p = a.list[j];
+  T r[SREC];
+#pragma unroll
+  for (int k = 0; k < SREC; ++k) r[k] = (T)0;
+  const double x = a.x[p], y = a.y[p];
+  if constexpr (KIND == LPM_K_SPHERE_BS || KIND == LPM_K_SPHERE_GREEN) {
+    const double z = a.z[p];
+    const double e = a.radius * a.radius - (x * x + y * y + z * z);
+    r[0] = (T)x;
+    r[1] = (T)y;
+    r[2] = (T)z;
+    r[3] = (T)e;
+    if constexpr (KIND == LPM_K_SPHERE_BS) {
+      r[4] = (T)(-a.q0[p] * a.area[p] / (2.0 * kPi * a.radius));
+      r[5] = idx_word<T>(p);
+    } else {
+      const double sc = (sizeof(T) == 4 ? kLn2 : 1.0) / (4.0 * kPi);
+      r[4] = (T)(-a.q0[p] * a.area[p] * sc);
+      r[5] = (T)(-a.q1[p] * a.area[p] * sc);
+      r[6] = idx_word<T>(p);
+    }
+  } else if constexpr (KIND == LPM_K_PLANE_BS || KIND == LPM_K_PLANE_GREEN) {
+    r[0] = (T)x;
+    r[1] = (T)y;
+    if constexpr (KIND == LPM_K_PLANE_BS)
+      r[2] = (T)(a.q0[p] * a.area[p] / (2.0 * kPi));
+    else
+      r[2] = (T)(a.q0[p] * a.area[p] * (sizeof(T) == 4 ? kLn2 : 1.0) / (4.0 * kPi));
+    r[3] = idx_word<T>(p);
+  } else {  // PSE
+    r[0] = (T)(x * a.sigma);
+    r[1] = (T)(y * a.sigma);
+    r[2] = (T)(a.q0[p] * a.area[p]);
+    r[3] = (T)a.area[p];
+    r[4] = idx_word<T>(p);
+  }
+  T* o = rec + (size_t)j * SREC;
+#pragma unroll
+  for (int k = 0; k < SREC; ++k) o[k] = r[k];
+}
+
+template <typename T, int KIND>
+__global__ __launch_bounds__(256) void pack_targets(PackArgs a, Vec4<T>* __restrict__ rec) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= a.count) return;
+  const int p = a.i0 + i;
+  const double x = a.x[p], y = a.y[p];
+  Vec4<T> t{(T)0, (T)0, (T)0, (T)0};
+  if constexpr (KIND == LPM_K_SPHERE_BS || KIND == LPM_K_SPHERE_GREEN) {
+    const double z = a.z[p];
+    t.a = (T)x;
+    t.b = (T)y;
+    t.c = (T)z;
+    t.d = (T)(a.radius * a.radius - (x * x + y * y + z * z));
+  } else if constexpr (KIND == LPM_K_PLANE_BS || KIND == LPM_K_PLANE_GREEN) {
+    t.a = (T)x;
+    t.b = (T)y;
+  } else {
+    t.a = (T)(x * a.sigma);
+    t.b = (T)(y * a.sigma);
+    t.c = (T)a.q0[p];
+  }
+  rec[i] = t;
+}
+
+// ----------------------------------------------------- green source totals
+// G_k = sum over sources of -q_k A / (4 pi) (sphere) in FP64, one block,
+// fixed order: lane-strided partials then a tree in LDS.
+__global__ __launch_bounds__(256) void source_totals(int ns, const int* __restrict__ list,
+                                                     const double* __restrict__ q0,
+                                                     const double* __restrict__ q1,
+                                                     const double* __restrict__ area,
+                                                     double* __restrict__ G) {
+  __shared__ double s0[256], s1[256];
+  double a0 = 0.0, a1 = 0.0;
+  for (int j = threadIdx.x; j < ns; j += 256) {
+    const int p = list[j];
+    a0 += -q0[p] * area[p] / (4.0 * kPi);
+    a1 += -q1[p] * area[p] / (4.0 * kPi);
+  }
+  s0[threadIdx.x] = a0;
+  s1[threadIdx.x] = a1;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) {
+      s0[threadIdx.x] += s0[threadIdx.x + w];
+      s1[threadIdx.x] += s1[threadIdx.x + w];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    G[0] = s0[0];
+    G[1] = s1[0];
+  }
+}
+
+// ---------------------------------------------------------------- finalize
+struct FinArgs {
+  const double* partial;
+  int nt, i0, nsplit;
+  const double *x, *y, *z, *q0, *q1, *area;
+  const int* active;
+  const double* G;
+  double radius, scale;
+  int fp32;
+  double *out0, *out1, *out2;
+};
+
+template <int KIND, int NO>
+__global__ __launch_bounds__(256) void finalize_kernel(FinArgs f) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= f.nt) return;
+  double A[NO];
+#pragma unroll
+  for (int k = 0; k < NO; ++k) A[k] = 0.0;
+  for (int s = 0; s < f.nsplit; ++s) {
+    const double* ps = f.partial + (size_t)s * NO * f.nt;
+#pragma unroll
+    for (int k = 0; k < NO; ++k) A[k] += ps[(size_t)k * f.nt + i];
+  }
+  const int p = f.i0 + i;
+  if constexpr (KIND == LPM_K_SPHERE_BS) {
+    const double x = f.x[p], y = f.y[p], z = f.z[p];
+    f.out0[p] = A[1] * z - A[2] * y;
+    f.out1[p] = A[2] * x - A[0] * z;
+    f.out2[p] = A[0] * y - A[1] * x;
+  } else if constexpr (KIND == LPM_K_PLANE_BS) {
+    f.out0[p] = -A[0];
+    f.out1[p] = A[1];
+  } else if constexpr (KIND == LPM_K_PSE8 || KIND == LPM_K_PSE2) {
+    f.out0[p] = A[0] * f.scale;
+  } else if constexpr (KIND == LPM_K_SPHERE_GREEN) {
+    if (f.fp32) {
+      const double act = f.active[p] ? 1.0 : 0.0;
+      const double gr = -f.q0[p] * f.area[p] / (4.0 * kPi) * act;
+      const double ga = -f.q1[p] * f.area[p] / (4.0 * kPi) * act;
+      f.out0[p] = A[0] - kLn2 * (f.G[0] - gr);
+      f.out1[p] = A[1] - kLn2 * (f.G[1] - ga);
+    } else {
+      f.out0[p] = A[0];
+      f.out1[p] = A[1];
+    }
+  } else {  // plane green
+    f.out0[p] = A[0];
+  }
+}
+
+// ------------------------------------------------------------ dispatching
+static int nout_of(int kind) {
+  switch (kind) {
+    case LPM_K_SPHERE_BS: return 3;
+    case LPM_K_PLANE_BS: return 2;
+    case LPM_K_SPHERE_GREEN: return 2;
+    default: return 1;
+  }
+}
+
+static Tuning default_tuning(int kind, int prec) {
+  Tuning t;
+  if (prec == LPM_PREC_FP32_KAHAN) {
+    t.r = 4;
+    t.kt = 64;
+  } else {
+    t.r = 2;
+    t.kt = 64;
+  }
+  (void)kind;
+  return t;
+}
+
+using KernelFn = void (*)(PairArgs);
+
+template <class P, typename T>
+static KernelFn pick_r(int r) {
+  constexpr bool C = sizeof(T) == 4;
+  switch (r) {
+    case 1: return pair_sum_kernel<P, T, 1, C>;
+    case 2: return pair_sum_kernel<P, T, 2, C>;
+    case 8: return pair_sum_kernel<P, T, 8, C>;
+    default: return pair_sum_kernel<P, T, 4, C>;
+  }
+}
+
+static KernelFn pick_kernel(int kind, int prec, int r, bool on_sphere) {
+  if (prec == LPM_PREC_FP32_KAHAN) {
+    switch (kind) {
+      case LPM_K_SPHERE_BS:
+        return on_sphere ? pick_r<SphereBS<float, false>, float>(r) : pick_r<SphereBS<float, true>, float>(r);
+      case LPM_K_PLANE_BS: return pick_r<PlaneBS<float>, float>(r);
+      case LPM_K_PSE8: return pick_r<PsePolicy<float, 8>, float>(r);
+      case LPM_K_PSE2: return pick_r<PsePolicy<float, 2>, float>(r);
+      case LPM_K_SPHERE_GREEN:
+        return on_sphere ? pick_r<SphereGreen<float, false>, float>(r) : pick_r<SphereGreen<float, true>, float>(r);
+      default: return pick_r<PlaneGreen<float>, float>(r);
+    }
+  }
+  switch (kind) {
+    case LPM_K_SPHERE_BS:
+      return on_sphere ? pick_r<SphereBS<double, false>, double>(r) : pick_r<SphereBS<double, true>, double>(r);
+    case LPM_K_PLANE_BS: return pick_r<PlaneBS<double>, double>(r);
+    case LPM_K_PSE8: return pick_r<PsePolicy<double, 8>, double>(r);
+    case LPM_K_PSE2: return pick_r<PsePolicy<double, 2>, double>(r);
+    case LPM_K_SPHERE_GREEN:
+      return on_sphere ? pick_r<SphereGreen<double, false>, double>(r) : pick_r<SphereGreen<double, true>, double>(r);
+    default: return pick_r<PlaneGreen<double>, double>(r);
+  }
+}
+
+template <typename T>
+static void launch_pack(int kind, const PackArgs& a, void* srec, void* trec, int ns, int nt,
+                        hipStream_t st) {
+  dim3 b(256);
+  dim3 gs((ns + 255) / 256), gt((nt + 255) / 256);
+  PackArgs as = a, at = a;
+  as.count = ns;
+  at.count = nt;
+#define LPM_PACK_CASE(K)                                                                    \
+  case K:                                                                                   \
+    if (ns > 0) hipLaunchKernelGGL((pack_sources<T, K>), gs, b, 0, st, as, (T*)srec);      \
+    if (nt > 0) hipLaunchKernelGGL((pack_targets<T, K>), gt, b, 0, st, at, (Vec4<T>*)trec); \
+    break;
+  switch (kind) {
+    LPM_PACK_CASE(LPM_K_SPHERE_BS)
+    LPM_PACK_CASE(LPM_K_PLANE_BS)
+    LPM_PACK_CASE(LPM_K_PSE8)
+    LPM_PACK_CASE(LPM_K_PSE2)
+    LPM_PACK_CASE(LPM_K_SPHERE_GREEN)
+    LPM_PACK_CASE(LPM_K_PLANE_GREEN)
+  }
+#undef LPM_PACK_CASE
+}
+
+static void launch_finalize(int kind, const FinArgs& f, hipStream_t st) {
+  dim3 b(256), g((f.nt + 255) / 256);
+  switch (kind) {
+    case LPM_K_SPHERE_BS: hipLaunchKernelGGL((finalize_kernel<LPM_K_SPHERE_BS, 3>), g, b, 0, st, f); break;
+    case LPM_K_PLANE_BS: hipLaunchKernelGGL((finalize_kernel<LPM_K_PLANE_BS, 2>), g, b, 0, st, f); break;
+    case LPM_K_PSE8: hipLaunchKernelGGL((finalize_kernel<LPM_K_PSE8, 1>), g, b, 0, st, f); break;
+    case LPM_K_PSE2: hipLaunchKernelGGL((finalize_kernel<LPM_K_PSE2, 1>), g, b, 0, st, f); break;
+    case LPM_K_SPHERE_GREEN: hipLaunchKernelGGL((finalize_kernel<LPM_K_SPHERE_GREEN, 2>), g, b, 0, st, f); break;
+    default: hipLaunchKernelGGL((finalize_kernel<LPM_K_PLANE_GREEN, 1>), g, b, 0, st, f); break;
+  }
+}
+
+static std::map<KernelFn, int>& occupancy_cache() {
+  static std::map<KernelFn, int> m;
+  return m;
+}
+
+static int blocks_per_cu(KernelFn fn) {
+  auto& m = occupancy_cache();
+  auto it = m.find(fn);
+  if (it != m.end()) return it->second;
+  int nb = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, reinterpret_cast<const void*>(fn), 256, 0) !=
+          hipSuccess ||
+      nb <= 0)
+    nb = 1;
+  m[fn] = nb;
+  return nb;
+}
+
+// Source splits: fill the chip in whole "rounds" of resident blocks.  All
+// blocks of a launch carry equal work, so the launch time is
+// ceil(B / slots) rounds; pick the fewest splits whose round efficiency
+// B / (rounds * slots) is at least 0.9 (or the best one available).
+static int choose_splits(int n_wtiles, int ns, int kt, int slots) {
+  const int nblk_t = (n_wtiles + 3) / 4;
+  const int min_chunk = std::max(kt, 256);
+  int max_split = std::max(1, ns / min_chunk);
+  max_split = std::min(max_split, 4096);
+  int best = 1;
+  double best_eff = -1.0;
+  for (int s = 1; s <= max_split; ++s) {
+    const double B = (double)nblk_t * s;
+    const double rounds = std::ceil(B / slots);
+    const double eff = B / (rounds * slots);
+    if (eff >= 0.9) return s;
+    if (eff > best_eff + 1e-9) {
+      best_eff = eff;
+      best = s;
+    }
+  }
+  return best;
+}
+
+static hipEvent_t take_event(lpm_ctx* ctx) {
+  if (!ctx->event_pool.empty()) {
+    hipEvent_t e = ctx->event_pool.back();
+    ctx->event_pool.pop_back();
+    return e;
+  }
+  hipEvent_t e = nullptr;
+  if (hipEventCreate(&e) != hipSuccess) return nullptr;
+  return e;
+}
+
+int collect_timing(lpm_ctx* ctx) {
+  if (ctx->pending.empty()) return LPM_OK;
+  LPM_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  for (auto& ep : ctx->pending) {
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, ep.a, ep.b) == hipSuccess) {
+      ctx->total_ms[ep.kind] += ms;
+      ctx->launches[ep.kind] += 1;
+    }
+    ctx->event_pool.push_back(ep.a);
+    ctx->event_pool.push_back(ep.b);
+  }
+  ctx->pending.clear();
+  return LPM_OK;
+}
+
+int run_pair_sum(lpm_ctx* ctx, const PairProblem& pp) {
+  const int kind = pp.kind;
+  const int prec = pp.precision;
+  if (kind < 0 || kind >= LPM_K_COUNT) return fail(ctx, LPM_ERR_ARG, "unknown pair kind");
+  if (prec != LPM_PREC_FP64 && prec != LPM_PREC_FP32_KAHAN)
+    return fail(ctx, LPM_ERR_ARG, "precision must be LPM_PREC_FP64 or LPM_PREC_FP32_KAHAN");
+  const int nt = pp.i1 - pp.i0;
+  if (nt <= 0) return LPM_OK;
+  const int pi = prec == LPM_PREC_FP32_KAHAN ? 1 : 0;
+  Tuning tu = default_tuning(kind, prec);
+  const Tuning& ov = ctx->tune[kind][pi];
+  if (ov.r) tu.r = ov.r;
+  if (ov.kt) tu.kt = ov.kt;
+  tu.splits = ov.splits;
+
+  const size_t wsz = prec == LPM_PREC_FP32_KAHAN ? 4 : 8;
+  const int no = nout_of(kind);
+  const int ns = pp.ns;
+  DevBuf& tb = pp.tgt_buf ? *pp.tgt_buf : ctx->tgt;
+  DevBuf& sb = pp.src_buf ? *pp.src_buf : ctx->src;
+  DevBuf& pb = pp.partial_buf ? *pp.partial_buf : ctx->partial;
+
+  const int per_wave = 64 * tu.r;
+  const int n_wtiles = (nt + per_wave - 1) / per_wave;
+  KernelFn fn = pick_kernel(kind, prec, tu.r, pp.on_sphere);
+  int nsplit = 1, chunk = std::max(ns, 1);
+  if (ns > 0) {
+    int want = tu.splits > 0 ? tu.splits : choose_splits(n_wtiles, ns, tu.kt, blocks_per_cu(fn) * ctx->num_cus);
+    chunk = (ns + want - 1) / want;
+    chunk = (chunk + tu.kt - 1) / tu.kt * tu.kt;
+    nsplit = (ns + chunk - 1) / chunk;
+  }
+
+  LPM_HIP(ctx, tb.ensure((size_t)nt * TREC * wsz));
+  LPM_HIP(ctx, sb.ensure((size_t)std::max(ns, 1) * SREC * wsz));
+  LPM_HIP(ctx, pb.ensure((size_t)nsplit * no * nt * sizeof(double)));
+
+  PackArgs pa{pp.x, pp.y, pp.z, pp.q0, pp.q1, pp.area, pp.src_list, 0, pp.i0, pp.radius, 1.0};
+  if (kind == LPM_K_PSE8 || kind == LPM_K_PSE2)
+    pa.sigma = (prec == LPM_PREC_FP32_KAHAN ? std::sqrt(kLog2e) : 1.0) / pp.eps;
+  if (prec == LPM_PREC_FP32_KAHAN)
+    launch_pack<float>(kind, pa, sb.p, tb.p, ns, nt, ctx->stream);
+  else
+    launch_pack<double>(kind, pa, sb.p, tb.p, ns, nt, ctx->stream);
+  LPM_HIP(ctx, hipGetLastError());
+
+  if (ns > 0) {
+    PairArgs a;
+    a.tgt = tb.p;
+    a.src = sb.p;
+    a.partial = pb.as<double>();
+    a.nt = nt;
+    a.tgt_idx0 = pp.i0;
+    a.ns = ns;
+    a.chunk = chunk;
+    a.nsplit = nsplit;
+    a.kt = tu.kt;
+    a.n_wtiles = n_wtiles;
+    const int nblk_t = (n_wtiles + 3) / 4;
+    dim3 grid((unsigned)(nblk_t * nsplit)), block(256);
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (ctx->timing) {
+      e0 = take_event(ctx);
+      e1 = take_event(ctx);
+      if (e0 && e1) LPM_HIP(ctx, hipEventRecord(e0, ctx->stream));
+    }
+    hipLaunchKernelGGL(fn, grid, block, 0, ctx->stream, a);
+    LPM_HIP(ctx, hipGetLastError());
+    if (ctx->timing && e0 && e1) {
+      LPM_HIP(ctx, hipEventRecord(e1, ctx->stream));
+      ctx->pending.push_back({e0, e1, kind});
+    }
+  } else {
+    nsplit = 1;
+    LPM_HIP(ctx, hipMemsetAsync(pb.p, 0, (size_t)no * nt * sizeof(double), ctx->stream));
+  }
+
+  FinArgs f;
+  f.partial = pb.as<double>();
+  f.nt = nt;
+  f.i0 = pp.i0;
+  f.nsplit = nsplit;
+  f.x = pp.x;
+  f.y = pp.y;
+  f.z = pp.z;
+  f.q0 = pp.q0;
+  f.q1 = pp.q1;
+  f.area = pp.area;
+  f.active = pp.active;
+  f.G = nullptr;
+  f.radius = pp.radius;
+  f.scale = 1.0;
+  f.fp32 = prec == LPM_PREC_FP32_KAHAN;
+  f.out0 = pp.out0;
+  f.out1 = pp.out1;
+  f.out2 = pp.out2;
+  if (kind == LPM_K_PSE8 || kind == LPM_K_PSE2) {
+    const double e4 = pp.eps * pp.eps * pp.eps * pp.eps;
+    f.scale = (kind == LPM_K_PSE8 ? 1.0 : 4.0) / (kPi * e4);
+  }
+  if (kind == LPM_K_SPHERE_GREEN && f.fp32) {
+    LPM_HIP(ctx, ctx->red.ensure(2 * sizeof(double)));
+    hipLaunchKernelGGL(source_totals, dim3(1), dim3(256), 0, ctx->stream, ns, pp.src_list, pp.q0,
+                       pp.q1, pp.area, ctx->red.as<double>());
+    f.G = ctx->red.as<double>();
+  }
+  launch_finalize(kind, f, ctx->stream);
+  LPM_HIP(ctx, hipGetLastError());
+  return LPM_OK;
+}
+
+// ------------------------------------------------------------- diagnostics
+// Deterministic two-pass FP64 reduction: fixed grid of 1024 blocks, each a
+// strided partial then an LDS tree; the last pass sums the 1024 partials.
+constexpr int kRedBlocks = 1024;
+
+__global__ __launch_bounds__(256) void diag_partial(int n, const double* __restrict__ u,
+                                                    const double* __restrict__ v,
+                                                    const double* __restrict__ w,
+                                                    const double* __restrict__ q,
+                                                    const double* __restrict__ area,
+                                                    const int* __restrict__ active,
+                                                    double* __restrict__ part) {
+  __shared__ double s[3][256];
+  double a0 = 0.0, a1 = 0.0, a2 = 0.0;
+  for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += kRedBlocks * 256) {
+    if (!active[i]) continue;
+    const double m = u[i] * u[i] + v[i] * v[i] + (w ? w[i] * w[i] : 0.0);
+    a0 += m * area[i];
+    a1 += q[i] * q[i] * area[i];
+    a2 += q[i] * area[i];
+  }
+  s[0][threadIdx.x] = a0;
+  s[1][threadIdx.x] = a1;
+  s[2][threadIdx.x] = a2;
+  __syncthreads();
+  for (int k = 128; k > 0; k >>= 1) {
+    if ((int)threadIdx.x < k)
+      for (int c = 0; c < 3; ++c) s[c][threadIdx.x] += s[c][threadIdx.x + k];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0)
+    for (int c = 0; c < 3; ++c) part[c * kRedBlocks + blockIdx.x] = s[c][0];
+}
+
+__global__ __launch_bounds__(256) void diag_final(const double* __restrict__ part,
+                                                  double* __restrict__ out) {
+  __shared__ double s[3][256];
+  for (int c = 0; c < 3; ++c) {
+    double a = 0.0;
+    for (int k = threadIdx.x; k < kRedBlocks; k += 256) a += part[c * kRedBlocks + k];
+    s[c][threadIdx.x] = a;
+  }
+  __syncthreads();
+  for (int k = 128; k > 0; k >>= 1) {
+    if ((int)threadIdx.x < k)
+      for (int c = 0; c < 3; ++c) s[c][threadIdx.x] += s[c][threadIdx.x + k];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    out[0] = 0.5 * s[0][0];
+    out[1] = 0.5 * s[1][0];
+    out[2] = s[2][0];
+  }
+}
+
+int run_diagnostics(lpm_ctx* ctx, int n, const double* u, const double* v, const double* w,
+                    const double* q, const double* area, const int* active, double out[3]) {
+  LPM_HIP(ctx, ctx->red.ensure((3 * kRedBlocks + 4) * sizeof(double)));
+  double* part = ctx->red.as<double>();
+  double* res = part + 3 * kRedBlocks;
+  hipLaunchKernelGGL(diag_partial, dim3(kRedBlocks), dim3(256), 0, ctx->stream, n, u, v, w, q,
+                     area, active, part);
+  hipLaunchKernelGGL(diag_final, dim3(1), dim3(256), 0, ctx->stream, part, res);
+  LPM_HIP(ctx, hipGetLastError());
+  LPM_HIP(ctx, hipMemcpyAsync(out, res, 3 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+  LPM_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return LPM_OK;
+}
+
+int upload_active_list(lpm_ctx* ctx, const int32_t* active_host, int n, DevBuf& list,
+                       DevBuf* flags, int* ns_out) {
+  std::vector<int> idx;
+  idx.reserve((size_t)n);
+  std::vector<int> fl((size_t)n);
+  for (int i = 0; i < n; ++i) {
+    fl[(size_t)i] = active_host[i] != 0;
+    if (fl[(size_t)i]) idx.push_back(i);
+  }
+  *ns_out = (int)idx.size();
+  LPM_HIP(ctx, list.ensure(std::max<size_t>(idx.size(), 1) * sizeof(int)));
+  if (!idx.empty())
+    LPM_HIP(ctx, hipMemcpyAsync(list.p, idx.data(), idx.size() * sizeof(int), hipMemcpyHostToDevice,
+                                ctx->stream));
+  if (flags) {
+    LPM_HIP(ctx, flags->ensure((size_t)std::max(n, 1) * sizeof(int)));
+    if (n > 0)
+      LPM_HIP(ctx, hipMemcpyAsync(flags->p, fl.data(), (size_t)n * sizeof(int), hipMemcpyHostToDevice,
+                                  ctx->stream));
+  }
+  // the host vectors die at return: make the copies complete first
+  LPM_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return LPM_OK;
+}
+
+}  // namespace lpm

@@ -1,0 +1,336 @@
+// pair_kernels.hpp -- tiled all-pairs direct-sum kernels for gfx950.
+//
+// One template covers every O(T*S) sum of the path.  A "pair policy" P
+// supplies the per-pair arithmetic; the template supplies the CDNA4 mapping:
+//
+//  * a wavefront owns 64*R consecutive targets: lane l holds targets
+//    t0 + r*64 + l (r < R) in registers, so each source value loaded is used
+//    by 64*R pairs;
+//  * sources are packed AoS records (8 words) read with wave-uniform
+//    addresses, which the compiler turns into scalar (SMEM) loads: the source
+//    operands of the pair arithmetic come straight from SGPRs and cost no
+//    LDS or VGPR traffic;
+//  * sources are walked in tiles of KT; inside a tile each target
+//    accumulates plainly in FP32, and the tile partial is folded into a
+//    (sum, compensation) pair with an error-free TwoSum (Knuth), so the
+//    compensated accumulation costs O(S/KT) per target;
+//  * the self pair j == i is excluded by index, exactly, but only tiles whose
+//    source-index range overlaps the wave's target range take the checked
+//    path (a wave-uniform branch);
+//  * a grid of (target tiles) x (source splits) fills the 256 CUs also when T
+//    is small; every split writes its FP64 partial per target and a
+//    finalize kernel reduces the splits in a fixed order (deterministic).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace lpm {
+
+// Source records: 8 words per source (FP32) or 8 doubles (FP64).
+//   sphere BS   : x y z e w2 idx . .     e = R^2-|x|^2, w2 = -zeta A/(2 pi R)
+//   plane  BS   : x y w idx . . . .      w = omega A / (2 pi)
+//   PSE         : x y fA A idx . . .     coordinates scaled (see PsePolicy)
+//   sphere green: x y z e gr ga idx .    g = -zeta A ln2/(4 pi) (FP32) or -zeta A/(4 pi) (FP64)
+//   plane  green: x y g idx . . . .      g = omega A ln2/(4 pi) (FP32) or omega A/(4 pi) (FP64)
+// Target records: 4 words per target: sphere {x y z e}, plane {x y . .},
+// PSE {x y f .}.
+constexpr int SREC = 8;
+constexpr int TREC = 4;
+
+template <typename T>
+struct Vec4 {
+  T a, b, c, d;
+};
+
+__device__ __forceinline__ float fast_rcp(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ __forceinline__ float fast_exp2(float x) { return __builtin_amdgcn_exp2f(x); }
+__device__ __forceinline__ float fast_log2(float x) { return __builtin_amdgcn_logf(x); }
+
+// ---------------------------------------------------------------- policies
+// A policy splits one pair into three phases so that the kernel can issue the
+// phase for all R targets of a lane back to back (independent instruction
+// streams hide the latency of the transcendental unit):
+//   arg   : differences and the argument of the transcendental (kept in W)
+//   trans : the one transcendental per pair (rcp, exp2 or log2)
+//   acc   : weight and accumulate into NOUT tile partials
+// W holds a policy's per-pair temporaries.
+
+template <typename T>
+struct W4 {
+  T a, b, c, z;
+};
+
+template <typename T, bool USE_E>
+struct SphereBS {
+  static constexpr int NOUT = 3;
+  static constexpr int IDX = 5;
+  // acc += s * (x_i - y_j),  s = w2 / (|x_i - y_j|^2 + e_i + e_j); with
+  // USE_E == false the points are on the sphere and e_i = e_j = 0.
+  __device__ __forceinline__ static void arg(const Vec4<T>& t, const T* s, W4<T>& w) {
+    w.a = t.a - s[0];
+    w.b = t.b - s[1];
+    w.c = t.c - s[2];
+    T r2;
+    if constexpr (USE_E)
+      r2 = fma(w.a, w.a, t.d + s[3]);
+    else
+      r2 = w.a * w.a;
+    r2 = fma(w.b, w.b, r2);
+    w.z = fma(w.c, w.c, r2);
+  }
+  __device__ __forceinline__ static T trans(const W4<T>& w) {
+    if constexpr (sizeof(T) == 4)
+      return fast_rcp(w.z);
+    else
+      return (T)1 / w.z;
+  }
+  __device__ __forceinline__ static void acc(const T* s, const W4<T>& w, T f, T a[NOUT]) {
+    T q = s[4] * f;
+    a[0] = fma(q, w.a, a[0]);
+    a[1] = fma(q, w.b, a[1]);
+    a[2] = fma(q, w.c, a[2]);
+  }
+};
+
+template <typename T>
+struct PlaneBS {
+  static constexpr int NOUT = 2;
+  static constexpr int IDX = 3;
+  __device__ __forceinline__ static void arg(const Vec4<T>& t, const T* s, W4<T>& w) {
+    w.a = t.a - s[0];
+    w.b = t.b - s[1];
+    w.z = fma(w.a, w.a, w.b * w.b);
+  }
+  __device__ __forceinline__ static T trans(const W4<T>& w) {
+    if constexpr (sizeof(T) == 4)
+      return fast_rcp(w.z);
+    else
+      return (T)1 / w.z;
+  }
+  __device__ __forceinline__ static void acc(const T* s, const W4<T>& w, T f, T a[NOUT]) {
+    T q = s[2] * f;
+    a[0] = fma(q, w.b, a[0]);
+    a[1] = fma(q, w.a, a[1]);
+  }
+};
+
+// PSE Laplacian.  FP32: coordinates are pre-scaled by sqrt(log2 e)/eps so that
+// q' = |dx|^2 = log2(e) rho^2, exp(-rho^2) = exp2(-q'), and the polynomial is
+// re-expressed in q'.  FP64: coordinates scaled by 1/eps, q = rho^2.
+template <typename T, int ORDER>
+struct PsePolicy {
+  static constexpr int NOUT = 1;
+  static constexpr int IDX = 4;
+  __device__ __forceinline__ static void arg(const Vec4<T>& t, const T* s, W4<T>& w) {
+    T dx = t.a - s[0], dy = t.b - s[1];
+    w.z = fma(dx, dx, dy * dy);
+    w.a = fma(-t.c, s[3], s[2]);  // f_j A_j - f_i A_j
+  }
+  __device__ __forceinline__ static T trans(const W4<T>& w) {
+    if constexpr (sizeof(T) == 4)
+      return fast_exp2(-w.z);
+    else
+      return exp(-w.z);
+  }
+  __device__ __forceinline__ static void acc(const T* s, const W4<T>& w, T e, T a[NOUT]) {
+    if constexpr (ORDER == 8) {
+      // 40 - 40 rho^2 + 10 rho^4 - (2/3) rho^6 in the (scaled) variable q
+      constexpr double L = sizeof(T) == 4 ? 1.4426950408889634 : 1.0;
+      const T c0 = (T)40.0, c1 = (T)(-40.0 / L), c2 = (T)(10.0 / (L * L)),
+              c3 = (T)(-(2.0 / 3.0) / (L * L * L));
+      const T q = w.z;
+      T p = fma(fma(fma(c3, q, c2), q, c1), q, c0);
+      a[0] = fma(p * e, w.a, a[0]);
+    } else {
+      a[0] = fma(e, w.a, a[0]);
+    }
+  }
+};
+
+template <typename T, bool USE_E>
+struct SphereGreen {
+  static constexpr int NOUT = 2;
+  static constexpr int IDX = 6;
+  __device__ __forceinline__ static void arg(const Vec4<T>& t, const T* s, W4<T>& w) {
+    T dx = t.a - s[0], dy = t.b - s[1], dz = t.c - s[2];
+    T r2;
+    if constexpr (USE_E)
+      r2 = fma(dx, dx, t.d + s[3]);
+    else
+      r2 = dx * dx;
+    r2 = fma(dy, dy, r2);
+    w.z = fma(dz, dz, r2);
+  }
+  __device__ __forceinline__ static T trans(const W4<T>& w) {
+    if constexpr (sizeof(T) == 4)
+      return fast_log2(w.z);  // log2(2 (R^2 - x.y))
+    else
+      return log((T)0.5 * w.z);  // ln(R^2 - x.y)
+  }
+  __device__ __forceinline__ static void acc(const T* s, const W4<T>& w, T lg, T a[NOUT]) {
+    a[0] = fma(lg, s[4], a[0]);
+    a[1] = fma(lg, s[5], a[1]);
+  }
+};
+
+template <typename T>
+struct PlaneGreen {
+  static constexpr int NOUT = 1;
+  static constexpr int IDX = 3;
+  __device__ __forceinline__ static void arg(const Vec4<T>& t, const T* s, W4<T>& w) {
+    T dx = t.a - s[0], dy = t.b - s[1];
+    w.z = fma(dx, dx, dy * dy);
+  }
+  __device__ __forceinline__ static T trans(const W4<T>& w) {
+    if constexpr (sizeof(T) == 4)
+      return fast_log2(w.z);
+    else
+      return log(w.z);
+  }
+  __device__ __forceinline__ static void acc(const T* s, const W4<T>& w, T lg, T a[NOUT]) {
+    a[0] = fma(lg, s[2], a[0]);
+  }
+};
+
+// ----------------------------------------------------------- the kernel
+struct PairArgs {
+  const void* tgt;   // nt target records (TREC words each)
+  const void* src;   // ns source records (SREC words each)
+  double* partial;   // [nsplit][NOUT][nt]
+  int nt;            // number of targets
+  int tgt_idx0;      // particle index of target 0 (self-pair exclusion)
+  int ns;            // number of sources
+  int chunk;         // sources per split (multiple of kt)
+  int nsplit;
+  int kt;            // sources per compensated tile
+  int n_wtiles;      // wave target tiles = ceil(nt / (64 R))
+};
+
+template <typename T>
+__device__ __forceinline__ int rec_index(const T* s, int slot) {
+  if constexpr (sizeof(T) == 4)
+    return __float_as_int(s[slot]);
+  else
+    return (int)s[slot];
+}
+
+// Error-free (sum, compensation) update with a tile partial p.
+template <typename T>
+__device__ __forceinline__ void two_sum_acc(T& S, T& C, T p) {
+  T t = S + p;
+  T bp = t - S;
+  T e = (S - (t - bp)) + (p - bp);
+  C += e;
+  S = t;
+}
+
+template <class P, typename T, int R, bool COMPENSATED>
+__global__ __launch_bounds__(256) void pair_sum_kernel(PairArgs a) {
+  constexpr int NO = P::NOUT;
+  const int lane = threadIdx.x & 63;
+  const int wid = threadIdx.x >> 6;
+  const int nblk_t = (a.n_wtiles + 3) >> 2;
+  const int split = blockIdx.x / nblk_t;
+  const int wtile = (blockIdx.x - split * nblk_t) * 4 + wid;
+  if (wtile >= a.n_wtiles) return;  // wave-uniform
+
+  const int t_begin = wtile * 64 * R;  // local target index of slot (r=0, lane 0)
+  const Vec4<T>* __restrict__ tg = reinterpret_cast<const Vec4<T>*>(a.tgt);
+  Vec4<T> t[R];
+  int tid[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    int i = t_begin + r * 64 + lane;
+    int ic = i < a.nt ? i : a.nt - 1;
+    t[r] = tg[ic];
+    tid[r] = a.tgt_idx0 + i;
+  }
+  const int wt_lo = a.tgt_idx0 + t_begin;
+  const int wt_hi = wt_lo + 64 * R - 1;
+
+  T S[R][NO], C[R][NO];
+#pragma unroll
+  for (int r = 0; r < R; ++r)
+#pragma unroll
+    for (int k = 0; k < NO; ++k) {
+      S[r][k] = (T)0;
+      C[r][k] = (T)0;
+    }
+
+  const T* __restrict__ src = reinterpret_cast<const T*>(a.src);
+  const int j_begin = split * a.chunk;
+  const int j_end = min(a.ns, j_begin + a.chunk);
+
+  for (int jt = j_begin; jt < j_end; jt += a.kt) {
+    const int jt_end = min(j_end, jt + a.kt);
+    const int idx_lo = rec_index(src + (size_t)jt * SREC, P::IDX);
+    const int idx_hi = rec_index(src + (size_t)(jt_end - 1) * SREC, P::IDX);
+    const bool checked = (idx_lo <= wt_hi) && (idx_hi >= wt_lo);
+    T p[R][NO];
+#pragma unroll
+    for (int r = 0; r < R; ++r)
+#pragma unroll
+      for (int k = 0; k < NO; ++k) p[r][k] = (T)0;
+
+    if (!checked) {
+#pragma unroll 2
+      for (int j = jt; j < jt_end; ++j) {
+        const T* s = src + (size_t)j * SREC;
+        T sv[SREC];
+#pragma unroll
+        for (int q = 0; q < SREC; ++q) sv[q] = s[q];
+        W4<T> w[R];
+        T f[R];
+#pragma unroll
+        for (int r = 0; r < R; ++r) P::arg(t[r], sv, w[r]);
+#pragma unroll
+        for (int r = 0; r < R; ++r) f[r] = P::trans(w[r]);
+#pragma unroll
+        for (int r = 0; r < R; ++r) P::acc(sv, w[r], f[r], p[r]);
+      }
+    } else {
+      for (int j = jt; j < jt_end; ++j) {
+        const T* s = src + (size_t)j * SREC;
+        T sv[SREC];
+#pragma unroll
+        for (int q = 0; q < SREC; ++q) sv[q] = s[q];
+        const int sidx = rec_index(sv, P::IDX);
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+          W4<T> w;
+          P::arg(t[r], sv, w);
+          T f = P::trans(w);
+          T tmp[NO];
+#pragma unroll
+          for (int k = 0; k < NO; ++k) tmp[k] = (T)0;
+          P::acc(sv, w, f, tmp);
+          const bool self = (sidx == tid[r]);
+#pragma unroll
+          for (int k = 0; k < NO; ++k) p[r][k] += self ? (T)0 : tmp[k];
+        }
+      }
+    }
+#pragma unroll
+    for (int r = 0; r < R; ++r)
+#pragma unroll
+      for (int k = 0; k < NO; ++k) {
+        if constexpr (COMPENSATED)
+          two_sum_acc(S[r][k], C[r][k], p[r][k]);
+        else
+          S[r][k] += p[r][k];
+      }
+  }
+
+  double* out = a.partial + (size_t)split * NO * a.nt;
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    int i = t_begin + r * 64 + lane;
+    if (i < a.nt) {
+#pragma unroll
+      for (int k = 0; k < NO; ++k) out[(size_t)k * a.nt + i] = (double)S[r][k] + (double)C[r][k];
+    }
+  }
+}
+
+}  // namespace lpm

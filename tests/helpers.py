@@ -1,0 +1,44 @@
+"""Shared inputs and error metrics for the tests."""
+import functools
+
+import numpy as np
+
+from lpmgpu import ic, mesh
+
+
+@functools.lru_cache(maxsize=None)
+def icos(level, cell_level=-1):
+    return mesh.icosahedral_mesh(level, cell_level=cell_level)
+
+
+@functools.lru_cache(maxsize=None)
+def cubed(level):
+    return mesh.cubed_sphere_mesh(level)
+
+
+def cfg0_inputs(level=4):
+    """Gaussian vortex 4 pi exp(-16 |x - x0|^2), x0 at lat 0.3, lon 0 (relative vorticity)."""
+    m = icos(level)
+    x0 = ic.lat_lon_point(0.3, 0.0)
+    zeta = ic.gaussian_vortex(m.x, m.y, m.z, x0, 4.0 * np.pi, 16.0)
+    return m, zeta
+
+
+def rh4_inputs(m):
+    return ic.rh4_vorticity(m.x, m.y, m.z)
+
+
+def rel_err_max(test, ref):
+    """Normwise max relative error max_i |d_i| / max_i |ref_i|; vectors are given
+    as a tuple of component arrays."""
+    t = np.atleast_2d(np.asarray(test, dtype=np.float64))
+    r = np.atleast_2d(np.asarray(ref, dtype=np.float64))
+    d = np.sqrt(np.sum((t - r) ** 2, axis=0))
+    m = np.sqrt(np.sum(r**2, axis=0))
+    return float(d.max() / m.max())
+
+
+def rel_err_rms(test, ref):
+    t = np.atleast_2d(np.asarray(test, dtype=np.float64))
+    r = np.atleast_2d(np.asarray(ref, dtype=np.float64))
+    return float(np.sqrt(np.sum((t - r) ** 2) / np.sum(r**2)))
