@@ -212,8 +212,10 @@ static int nout_of(int kind) {
 static Tuning default_tuning(int kind, int prec) {
   Tuning t;
   if (prec == LPM_PREC_FP32_KAHAN) {
+    // measured on MI355X at icosahedral L7/L8: R = 4 beats 2 and 8 (occupancy
+    // 6 waves/SIMD at 74 VGPRs), longer compensated tiles cost less
     t.r = 4;
-    t.kt = 64;
+    t.kt = 256;
   } else {
     t.r = 2;
     t.kt = 64;

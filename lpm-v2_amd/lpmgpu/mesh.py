@@ -21,10 +21,10 @@ Face areas are the reference's `TriFaceArea` / `QuadFaceArea`
 triangle (v_k, centre, v_k+1), with L'Huilier's formula
 (`SphereTriAreaVector`, SphereGeometry.f90:347-366).
 
-Particle order is [vertices..., face centres...]; vertices are passive
-(area 0) and face centres are active, as in the reference
-(Faces.f90:840-850; Particles.f90).  The reference interleaves vertices and
-centres as they are created; the sums do not depend on the order.
+Particles are numbered in the reference's creation order (seed vertices,
+seed centres, then per level and face: new edge midpoints, new child
+centres); leaf-face centres are active, every other particle is a passive
+vertex with area 0 (Faces.f90:840-850; Particles.f90).
 """
 from __future__ import annotations
 
@@ -48,6 +48,7 @@ class ParticleMesh:
     level: int
     radius: float = 1.0
     faces: np.ndarray | None = None  # leaf-face vertex indices
+    face_centre: np.ndarray | None = None  # centre particle of each leaf face
     face_cell: np.ndarray | None = None  # ancestor face id at `cell_level`
     cell_level: int = -1
     max_edge_length: float = 0.0
@@ -116,53 +117,96 @@ def _icosahedron():
     return v, f
 
 
-def _edge_midpoints(verts, faces, nvert_per_face, sphere, radius):
+def _divide_level(pos, faces, centre, K, sphere, radius):
+    """One uniform division of every leaf face, numbering new particles in the
+    reference's creation order: faces are divided in order (PolyMesh2d.f90:179-189);
+    for each face its edges are visited in slot order k = (v_k, v_k+1) and an
+    edge's midpoint is inserted the first time it is met (Faces.f90:727-758,
+    divideLinearEdge Edges.f90:567-617); then the new child centres are
+    inserted, children 1..3 for triangles (child 4 keeps the parent's centre
+    particle, Faces.f90:836-850) and children 1..4 for quadrilaterals (the
+    parent centre becomes a passive vertex, Faces.f90:640-655)."""
     nf = faces.shape[0]
-    k = nvert_per_face
+    n0 = pos.shape[0]
     a = faces
     b = np.roll(faces, -1, axis=1)
-    lo = np.minimum(a, b).ravel()
-    hi = np.maximum(a, b).ravel()
-    nv = verts.shape[0]
-    key = lo.astype(np.int64) * nv + hi
-    uniq, inv = np.unique(key, return_inverse=True)
-    ea = uniq // nv
-    eb = uniq % nv
-    mid = 0.5 * (verts[ea] + verts[eb])
+    lo = np.minimum(a, b).ravel().astype(np.int64)
+    hi = np.maximum(a, b).ravel().astype(np.int64)
+    key = lo * (n0 + 1) + hi
+    uniq, first, inv = np.unique(key, return_index=True, return_inverse=True)
+    n_mid = uniq.shape[0]
+    n_cen = 3 if K == 3 else 4
+    # events: midpoint of edge e at (face, slot) of its first encounter, then centres
+    ev_face = np.concatenate([first // K, np.repeat(np.arange(nf), n_cen)])
+    ev_sub = np.concatenate([first % K, np.tile(np.arange(n_cen), nf) + K])
+    order = np.lexsort((ev_sub, ev_face))
+    new_index = np.empty(order.shape[0], np.int64)
+    new_index[order] = n0 + np.arange(order.shape[0])
+    mid_idx = new_index[:n_mid][inv].reshape(nf, K)  # midpoint of edge slot k of face f
+    cen_idx = new_index[n_mid:].reshape(nf, n_cen)
+    ea = uniq // (n0 + 1)
+    eb = uniq % (n0 + 1)
+    mid = 0.5 * (pos[ea] + pos[eb])
     if sphere:
         mid = _normalize(mid, radius)
-    mid_idx = nv + inv.reshape(nf, k)
-    return np.concatenate([verts, mid], axis=0), mid_idx
+    newpos = np.empty((order.shape[0], 3))
+    newpos[new_index[:n_mid] - n0] = mid
+    pos = np.concatenate([pos, newpos], axis=0)
+    if K == 3:
+        v1, v2, v3 = faces[:, 0], faces[:, 1], faces[:, 2]
+        m1, m2, m3 = mid_idx[:, 0], mid_idx[:, 1], mid_idx[:, 2]
+        ch = [np.stack([v1, m1, m3], 1), np.stack([m1, v2, m2], 1), np.stack([m3, m2, v3], 1),
+              np.stack([m2, m3, m1], 1)]
+        for c in range(3):
+            cc = _normalize(pos[ch[c]].sum(axis=1) / 3.0, radius) if sphere else pos[ch[c]].sum(axis=1) / 3.0
+            pos[cen_idx[:, c]] = cc
+        newcentre = np.stack([cen_idx[:, 0], cen_idx[:, 1], cen_idx[:, 2], centre], 1).reshape(-1)
+    else:
+        v1, v2, v3, v4 = (faces[:, k] for k in range(4))
+        m1, m2, m3, m4 = (mid_idx[:, k] for k in range(4))
+        C = centre
+        ch = [np.stack([v1, m1, C, m4], 1), np.stack([m1, v2, m2, C], 1), np.stack([C, m2, v3, m3], 1),
+              np.stack([m4, C, m3, v4], 1)]
+        for c in range(4):
+            cc = pos[ch[c]].sum(axis=1) / 4.0
+            pos[cen_idx[:, c]] = _normalize(cc, radius) if sphere else cc
+        newcentre = cen_idx.reshape(-1)
+    faces = np.stack(ch, axis=1).reshape(-1, K)
+    return pos, faces, newcentre
+
+
+def _build(seed_pos, seed_faces, seed_centre_pos, level, K, sphere, radius, cell_level):
+    nv = seed_pos.shape[0]
+    pos = np.concatenate([seed_pos, seed_centre_pos], axis=0)
+    faces = seed_faces.astype(np.int64)
+    centre = nv + np.arange(faces.shape[0])
+    cell = np.arange(faces.shape[0]) if cell_level == 0 else np.full(faces.shape[0], -1)
+    for lev in range(level):
+        pos, faces, centre = _divide_level(pos, faces, centre, K, sphere, radius)
+        cell = np.arange(faces.shape[0]) if lev + 1 == cell_level else np.repeat(cell, 4)
+    cpos = pos[centre]
+    area = np.zeros(faces.shape[0])
+    for k in range(K):
+        va = pos[faces[:, k]]
+        vb = pos[faces[:, (k + 1) % K]]
+        if sphere:
+            area += sphere_tri_area(va, cpos, vb, radius)
+        else:
+            # TriArea(v1, centre, v2), PlaneGeometry.f90:109-113
+            area += 0.5 * np.abs((va[:, 0] - cpos[:, 0]) * (vb[:, 1] - cpos[:, 1])
+                                 - (vb[:, 0] - cpos[:, 0]) * (va[:, 1] - cpos[:, 1]))
+    return pos, faces, centre, area, cell
 
 
 def icosahedral_mesh(level: int, radius: float = 1.0, cell_level: int = -1) -> ParticleMesh:
     """Icosahedral triangle mesh at uniform nest `level`: V = 10*4^L + 2 passive
-    vertices, F = 20*4^L active centres (PolyMesh2d.f90:737-738, 761-762)."""
+    vertices, F = 20*4^L active centres (PolyMesh2d.f90:737-738, 761-762).  The
+    seed's 20 centres are SphereTriCenter of their vertices (PolyMesh2d.f90:875-879)."""
     verts, faces = _icosahedron()
     verts = verts * radius
     centres = _normalize(verts[faces].sum(axis=1) / 3.0, radius)
-    cell = np.arange(faces.shape[0]) if cell_level == 0 else np.full(faces.shape[0], -1)
-    for lev in range(level):
-        verts, m = _edge_midpoints(verts, faces, 3, True, radius)
-        a, b, c = faces[:, 0], faces[:, 1], faces[:, 2]
-        mab, mbc, mca = m[:, 0], m[:, 1], m[:, 2]
-        c1 = np.stack([a, mab, mca], 1)
-        c2 = np.stack([mab, b, mbc], 1)
-        c3 = np.stack([mca, mbc, c], 1)
-        c4 = np.stack([mbc, mca, mab], 1)
-        newc = [
-            _normalize(verts[cc].sum(axis=1) / 3.0, radius) for cc in (c1, c2, c3)
-        ] + [centres]
-        faces = np.stack([c1, c2, c3, c4], axis=1).reshape(-1, 3)
-        centres = np.stack(newc, axis=1).reshape(-1, 3)
-        if lev + 1 == cell_level:
-            cell = np.arange(faces.shape[0])
-        else:
-            cell = np.repeat(cell, 4)
-    area = np.zeros(faces.shape[0])
-    for k in range(3):
-        area += sphere_tri_area(verts[faces[:, k]], centres, verts[faces[:, (k + 1) % 3]], radius)
-    return _assemble(verts, centres, area, faces, "icos", level, radius, cell, cell_level, sphere=True)
+    pos, f, c, area, cell = _build(verts, faces, centres, level, 3, True, radius, cell_level)
+    return _assemble(pos, f, c, area, "icos", level, radius, cell, cell_level, sphere=True)
 
 
 def _cube_seed():
@@ -174,43 +218,14 @@ def _cube_seed():
     return corners, quads
 
 
-def _quad_divide(verts, faces, centres, sphere, radius):
-    """One uniform quad division; parent centres become vertices."""
-    nv0 = verts.shape[0]
-    verts, m = _edge_midpoints(verts, faces, 4, sphere, radius)
-    nf = faces.shape[0]
-    cidx = verts.shape[0] + np.arange(nf)
-    verts = np.concatenate([verts, centres], axis=0)
-    v0, v1, v2, v3 = (faces[:, k] for k in range(4))
-    m0, m1, m2, m3 = (m[:, k] for k in range(4))  # m_k on edge (v_k, v_k+1)
-    ch = [
-        np.stack([v0, m0, cidx, m3], 1),
-        np.stack([m0, v1, m1, cidx], 1),
-        np.stack([cidx, m1, v2, m2], 1),
-        np.stack([m3, cidx, m2, v3], 1),
-    ]
-    faces = np.stack(ch, axis=1).reshape(-1, 4)
-    cen = verts[faces].sum(axis=1) / 4.0
-    if sphere:
-        cen = _normalize(cen, radius)
-    del nv0
-    return verts, faces, cen
-
-
 def cubed_sphere_mesh(level: int, radius: float = 1.0, cell_level: int = -1) -> ParticleMesh:
     """Cubed-sphere quad mesh at uniform nest `level`: V = 6*4^L + 2 passive
     vertices, F = 6*4^L active centres (PolyMesh2d.f90:739-740, 763-764)."""
     verts, faces = _cube_seed()
     verts = verts * radius
     centres = _normalize(verts[faces].sum(axis=1) / 4.0, radius)
-    cell = np.arange(6) if cell_level == 0 else np.full(6, -1)
-    for lev in range(level):
-        verts, faces, centres = _quad_divide(verts, faces, centres, True, radius)
-        cell = np.arange(faces.shape[0]) if lev + 1 == cell_level else np.repeat(cell, 4)
-    area = np.zeros(faces.shape[0])
-    for k in range(4):
-        area += sphere_tri_area(verts[faces[:, k]], centres, verts[faces[:, (k + 1) % 4]], radius)
-    return _assemble(verts, centres, area, faces, "cubed", level, radius, cell, cell_level, sphere=True)
+    pos, f, c, area, cell = _build(verts, faces, centres, level, 4, True, radius, cell_level)
+    return _assemble(pos, f, c, area, "cubed", level, radius, cell, cell_level, sphere=True)
 
 
 def quad_rect_mesh(level: int, half_width: float = 1.0) -> ParticleMesh:
@@ -221,45 +236,36 @@ def quad_rect_mesh(level: int, half_width: float = 1.0) -> ParticleMesh:
     faces = np.array([[0, 1, 8, 7], [1, 2, 3, 8], [8, 3, 4, 5], [7, 8, 5, 6]])
     verts3 = np.concatenate([verts, np.zeros((verts.shape[0], 1))], axis=1)
     centres = verts3[faces].sum(axis=1) / 4.0
-    for _ in range(level):
-        verts3, faces, centres = _quad_divide(verts3, faces, centres, False, 1.0)
-    area = np.zeros(faces.shape[0])
-    for k in range(4):
-        a = verts3[faces[:, k]]
-        b = verts3[faces[:, (k + 1) % 4]]
-        # TriArea(v1, centre, v2), PlaneGeometry.f90:109-113
-        area += 0.5 * np.abs((a[:, 0] - centres[:, 0]) * (b[:, 1] - centres[:, 1]) - (b[:, 0] - centres[:, 0]) * (a[:, 1] - centres[:, 1]))
-    cell = np.full(faces.shape[0], -1)
-    return _assemble(verts3, centres, area, faces, "quadrect", level, 1.0, cell, -1, sphere=False)
+    pos, f, c, area, cell = _build(verts3, faces, centres, level, 4, False, 1.0, -1)
+    return _assemble(pos, f, c, area, "quadrect", level, 1.0, cell, -1, sphere=False)
 
 
-def _assemble(verts, centres, area, faces, kind, level, radius, cell, cell_level, sphere):
-    nv = verts.shape[0]
-    nf = centres.shape[0]
-    xyz = np.concatenate([verts, centres], axis=0)
-    ar = np.concatenate([np.zeros(nv), area])
-    act = np.concatenate([np.zeros(nv, np.int32), np.ones(nf, np.int32)])
+def _assemble(pos, faces, centre, area, kind, level, radius, cell, cell_level, sphere):
+    n = pos.shape[0]
+    ar = np.zeros(n)
+    ar[centre] = area
+    act = np.zeros(n, np.int32)
+    act[centre] = 1
     # maximum leaf-edge length (Edges.f90:260-275): arc on the sphere, chord on the plane
-    k = faces.shape[1]
-    a = verts[faces].reshape(-1, 3)
-    b = verts[np.roll(faces, -1, axis=1)].reshape(-1, 3)
+    a = pos[faces].reshape(-1, 3)
+    b = pos[np.roll(faces, -1, axis=1)].reshape(-1, 3)
     if sphere:
         maxlen = float(np.max(_arc(a, b)) * radius)
     else:
         maxlen = float(np.max(np.sqrt(np.sum((a - b) ** 2, axis=1))))
-    del k
     return ParticleMesh(
-        x=np.ascontiguousarray(xyz[:, 0]),
-        y=np.ascontiguousarray(xyz[:, 1]),
-        z=np.ascontiguousarray(xyz[:, 2]),
+        x=np.ascontiguousarray(pos[:, 0]),
+        y=np.ascontiguousarray(pos[:, 1]),
+        z=np.ascontiguousarray(pos[:, 2]),
         area=ar,
         active=act,
-        n_vertices=nv,
-        n_faces=nf,
+        n_vertices=n - centre.shape[0],
+        n_faces=centre.shape[0],
         kind=kind,
         level=level,
         radius=radius,
         faces=faces,
+        face_centre=centre,
         face_cell=cell,
         cell_level=cell_level,
         max_edge_length=maxlen,
