@@ -178,6 +178,40 @@ int lpm_plane_solver_diagnostics(lpm_plane_solver *solver, double *ke, double *e
                                  double *circulation);
 int lpm_plane_solver_delete(lpm_plane_solver *solver);
 
+/* ------------------------------------------------- batched sphere problems -- */
+
+/* Many independent sphere BVE problems of uneven size advanced together: one
+ * persistent launch drains a work list of (problem, target tile, source
+ * split) items per velocity evaluation, and each RK4 stage can be replayed
+ * as a captured hipGraph (LPM_SOLVER_USE_GRAPH).  Problem k owns particles
+ * [offsets[k], offsets[k+1]) of the concatenated arrays and has its own
+ * radius[k] and rotation_rate[k]; problems never interact.  Each problem's
+ * step is the BVESolver step (SphereBVESolver.f90:219-353); flags as for
+ * lpm_bve_solver_new.  No counterpart in the reference (which runs one
+ * problem per program). */
+typedef struct lpm_bve_batch lpm_bve_batch;
+
+int lpm_bve_batch_new(lpm_ctx *ctx, lpm_bve_batch **batch, int n_inst, const int64_t *offsets,
+                      const double *x, const double *y, const double *z, const double *relvort,
+                      const double *area, const int32_t *active, const double *radius,
+                      const double *rotation_rate, int precision, int flags);
+int lpm_bve_batch_steps(lpm_bve_batch *batch, double dt, int nsteps);
+int lpm_bve_batch_get_state(lpm_bve_batch *batch, double *x, double *y, double *z,
+                            double *relvort, double *u, double *v, double *w);
+/* Tile shape: targets per lane (2, 4, 8), sources per compensated tile and
+ * sources per work item. */
+int lpm_bve_batch_set_tuning(lpm_bve_batch *batch, int targets_per_lane, int tile_sources,
+                             int chunk_sources);
+int lpm_bve_batch_get_tuning(lpm_bve_batch *batch, int *targets_per_lane, int *tile_sources,
+                             int *chunk_sources, int *n_items);
+/* Autotuner: times one velocity evaluation for each candidate tile shape and
+ * keeps the fastest (any output pointer may be NULL). */
+int lpm_bve_batch_autotune(lpm_bve_batch *batch, int *targets_per_lane, int *tile_sources,
+                           int *chunk_sources, double *best_ms);
+/* Pair interactions of one velocity evaluation (sum over problems of T*S - S). */
+int lpm_bve_batch_pairs(lpm_bve_batch *batch, int64_t *pairs_per_eval);
+int lpm_bve_batch_delete(lpm_bve_batch *batch);
+
 /* ----------------------------------------------------- timing / tuning -- */
 
 /* When enabled, every pair-kernel launch is bracketed by HIP events on the

@@ -118,6 +118,22 @@ struct PairProblem {
 // Enqueue the whole sum (pack, pair kernel, finalize) on ctx->stream.
 int run_pair_sum(lpm_ctx* ctx, const PairProblem& pp);
 
+// Record packing (FP64 SoA -> FP32/FP64 pair records).  prad, when not null,
+// gives a per-particle sphere radius (batched problems).
+struct PackArgs {
+  const double *x, *y, *z, *q0, *q1, *area;
+  const double* prad;
+  const int* list;
+  int count;  // records to write
+  int i0;     // first target particle (targets)
+  double radius, sigma;
+};
+void launch_pack(int kind, int precision, const PackArgs& a, void* srec, void* trec, int ns, int nt,
+                 hipStream_t st);
+Tuning default_tuning(int kind, int prec);
+int blocks_per_cu(const void* kernel);
+hipEvent_t take_event(lpm_ctx* ctx);
+
 // Resolve queued timing events (synchronizes the stream).
 int collect_timing(lpm_ctx* ctx);
 

@@ -32,13 +32,6 @@ int hip_fail(lpm_ctx* ctx, hipError_t e, const char* what) {
 }
 
 // ------------------------------------------------------------------ packing
-struct PackArgs {
-  const double *x, *y, *z, *q0, *q1, *area;
-  const int* list;
-  int count;   // records to write
-  int i0;      // first target particle (targets)
-  double radius, sigma;
-};
 
 template <typename T>
 __device__ __forceinline__ T idx_word(int p) {
@@ -59,13 +52,14 @@ __global__ __launch_bounds__(256) void pack_sources(PackArgs a, T* __restrict__ 
   const double x = a.x[p], y = a.y[p];
   if constexpr (KIND == LPM_K_SPHERE_BS || KIND == LPM_K_SPHERE_GREEN) {
     const double z = a.z[p];
-    const double e = a.radius * a.radius - (x * x + y * y + z * z);
+    const double rad = a.prad ? a.prad[p] : a.radius;
+    const double e = rad * rad - (x * x + y * y + z * z);
     r[0] = (T)x;
     r[1] = (T)y;
     r[2] = (T)z;
     r[3] = (T)e;
     if constexpr (KIND == LPM_K_SPHERE_BS) {
-      r[4] = (T)(-a.q0[p] * a.area[p] / (2.0 * kPi * a.radius));
+      r[4] = (T)(-a.q0[p] * a.area[p] / (2.0 * kPi * rad));
       r[5] = idx_word<T>(p);
     } else {
       const double sc = (sizeof(T) == 4 ? kLn2 : 1.0) / (4.0 * kPi);
@@ -102,10 +96,11 @@ __global__ __launch_bounds__(256) void pack_targets(PackArgs a, Vec4<T>* __restr
   Vec4<T> t{(T)0, (T)0, (T)0, (T)0};
   if constexpr (KIND == LPM_K_SPHERE_BS || KIND == LPM_K_SPHERE_GREEN) {
     const double z = a.z[p];
+    const double rad = a.prad ? a.prad[p] : a.radius;
     t.a = (T)x;
     t.b = (T)y;
     t.c = (T)z;
-    t.d = (T)(a.radius * a.radius - (x * x + y * y + z * z));
+    t.d = (T)(rad * rad - (x * x + y * y + z * z));
   } else if constexpr (KIND == LPM_K_PLANE_BS || KIND == LPM_K_PLANE_GREEN) {
     t.a = (T)x;
     t.b = (T)y;
@@ -209,7 +204,7 @@ static int nout_of(int kind) {
   }
 }
 
-static Tuning default_tuning(int kind, int prec) {
+Tuning default_tuning(int kind, int prec) {
   Tuning t;
   if (prec == LPM_PREC_FP32_KAHAN) {
     // measured on MI355X at icosahedral L7/L8: R = 4 beats 2 and 8 (occupancy
@@ -263,8 +258,8 @@ static KernelFn pick_kernel(int kind, int prec, int r, bool on_sphere) {
 }
 
 template <typename T>
-static void launch_pack(int kind, const PackArgs& a, void* srec, void* trec, int ns, int nt,
-                        hipStream_t st) {
+static void launch_pack_t(int kind, const PackArgs& a, void* srec, void* trec, int ns, int nt,
+                          hipStream_t st) {
   dim3 b(256);
   dim3 gs((ns + 255) / 256), gt((nt + 255) / 256);
   PackArgs as = a, at = a;
@@ -286,6 +281,14 @@ static void launch_pack(int kind, const PackArgs& a, void* srec, void* trec, int
 #undef LPM_PACK_CASE
 }
 
+void launch_pack(int kind, int precision, const PackArgs& a, void* srec, void* trec, int ns, int nt,
+                 hipStream_t st) {
+  if (precision == LPM_PREC_FP32_KAHAN)
+    launch_pack_t<float>(kind, a, srec, trec, ns, nt, st);
+  else
+    launch_pack_t<double>(kind, a, srec, trec, ns, nt, st);
+}
+
 static void launch_finalize(int kind, const FinArgs& f, hipStream_t st) {
   dim3 b(256), g((f.nt + 255) / 256);
   switch (kind) {
@@ -303,7 +306,8 @@ static std::map<KernelFn, int>& occupancy_cache() {
   return m;
 }
 
-static int blocks_per_cu(KernelFn fn) {
+int blocks_per_cu(const void* kfn) {
+  KernelFn fn = reinterpret_cast<KernelFn>(const_cast<void*>(kfn));
   auto& m = occupancy_cache();
   auto it = m.find(fn);
   if (it != m.end()) return it->second;
@@ -340,7 +344,7 @@ static int choose_splits(int n_wtiles, int ns, int kt, int slots) {
   return best;
 }
 
-static hipEvent_t take_event(lpm_ctx* ctx) {
+hipEvent_t take_event(lpm_ctx* ctx) {
   if (!ctx->event_pool.empty()) {
     hipEvent_t e = ctx->event_pool.back();
     ctx->event_pool.pop_back();
@@ -394,7 +398,8 @@ int run_pair_sum(lpm_ctx* ctx, const PairProblem& pp) {
   KernelFn fn = pick_kernel(kind, prec, tu.r, pp.on_sphere);
   int nsplit = 1, chunk = std::max(ns, 1);
   if (ns > 0) {
-    int want = tu.splits > 0 ? tu.splits : choose_splits(n_wtiles, ns, tu.kt, blocks_per_cu(fn) * ctx->num_cus);
+    int want = tu.splits > 0 ? tu.splits
+                             : choose_splits(n_wtiles, ns, tu.kt, blocks_per_cu((const void*)fn) * ctx->num_cus);
     chunk = (ns + want - 1) / want;
     chunk = (chunk + tu.kt - 1) / tu.kt * tu.kt;
     nsplit = (ns + chunk - 1) / chunk;
@@ -404,13 +409,10 @@ int run_pair_sum(lpm_ctx* ctx, const PairProblem& pp) {
   LPM_HIP(ctx, sb.ensure((size_t)std::max(ns, 1) * SREC * wsz));
   LPM_HIP(ctx, pb.ensure((size_t)nsplit * no * nt * sizeof(double)));
 
-  PackArgs pa{pp.x, pp.y, pp.z, pp.q0, pp.q1, pp.area, pp.src_list, 0, pp.i0, pp.radius, 1.0};
+  PackArgs pa{pp.x, pp.y, pp.z, pp.q0, pp.q1, pp.area, nullptr, pp.src_list, 0, pp.i0, pp.radius, 1.0};
   if (kind == LPM_K_PSE8 || kind == LPM_K_PSE2)
     pa.sigma = (prec == LPM_PREC_FP32_KAHAN ? std::sqrt(kLog2e) : 1.0) / pp.eps;
-  if (prec == LPM_PREC_FP32_KAHAN)
-    launch_pack<float>(kind, pa, sb.p, tb.p, ns, nt, ctx->stream);
-  else
-    launch_pack<double>(kind, pa, sb.p, tb.p, ns, nt, ctx->stream);
+  launch_pack(kind, prec, pa, sb.p, tb.p, ns, nt, ctx->stream);
   LPM_HIP(ctx, hipGetLastError());
 
   if (ns > 0) {

@@ -225,28 +225,26 @@ __device__ __forceinline__ void two_sum_acc(T& S, T& C, T p) {
   S = t;
 }
 
+// One wavefront's work: targets [t_begin, t_begin + 64 R) of the record array
+// tg[0, nt) (particle index tgt_idx0 + local) against sources [j_begin, j_end)
+// of src; FP64 results go to out[k * out_stride + i] for valid targets i.
 template <class P, typename T, int R, bool COMPENSATED>
-__global__ __launch_bounds__(256) void pair_sum_kernel(PairArgs a) {
+__device__ __forceinline__ void pair_tile(const Vec4<T>* __restrict__ tg, int nt, int t_begin,
+                                          int tgt_idx0, const T* __restrict__ src, int j_begin,
+                                          int j_end, int kt, double* __restrict__ out,
+                                          int out_stride) {
   constexpr int NO = P::NOUT;
   const int lane = threadIdx.x & 63;
-  const int wid = threadIdx.x >> 6;
-  const int nblk_t = (a.n_wtiles + 3) >> 2;
-  const int split = blockIdx.x / nblk_t;
-  const int wtile = (blockIdx.x - split * nblk_t) * 4 + wid;
-  if (wtile >= a.n_wtiles) return;  // wave-uniform
-
-  const int t_begin = wtile * 64 * R;  // local target index of slot (r=0, lane 0)
-  const Vec4<T>* __restrict__ tg = reinterpret_cast<const Vec4<T>*>(a.tgt);
   Vec4<T> t[R];
   int tid[R];
 #pragma unroll
   for (int r = 0; r < R; ++r) {
     int i = t_begin + r * 64 + lane;
-    int ic = i < a.nt ? i : a.nt - 1;
+    int ic = i < nt ? i : nt - 1;
     t[r] = tg[ic];
-    tid[r] = a.tgt_idx0 + i;
+    tid[r] = tgt_idx0 + i;
   }
-  const int wt_lo = a.tgt_idx0 + t_begin;
+  const int wt_lo = tgt_idx0 + t_begin;
   const int wt_hi = wt_lo + 64 * R - 1;
 
   T S[R][NO], C[R][NO];
@@ -258,12 +256,8 @@ __global__ __launch_bounds__(256) void pair_sum_kernel(PairArgs a) {
       C[r][k] = (T)0;
     }
 
-  const T* __restrict__ src = reinterpret_cast<const T*>(a.src);
-  const int j_begin = split * a.chunk;
-  const int j_end = min(a.ns, j_begin + a.chunk);
-
-  for (int jt = j_begin; jt < j_end; jt += a.kt) {
-    const int jt_end = min(j_end, jt + a.kt);
+  for (int jt = j_begin; jt < j_end; jt += kt) {
+    const int jt_end = min(j_end, jt + kt);
     const int idx_lo = rec_index(src + (size_t)jt * SREC, P::IDX);
     const int idx_hi = rec_index(src + (size_t)(jt_end - 1) * SREC, P::IDX);
     const bool checked = (idx_lo <= wt_hi) && (idx_hi >= wt_lo);
@@ -322,14 +316,76 @@ __global__ __launch_bounds__(256) void pair_sum_kernel(PairArgs a) {
       }
   }
 
-  double* out = a.partial + (size_t)split * NO * a.nt;
 #pragma unroll
   for (int r = 0; r < R; ++r) {
     int i = t_begin + r * 64 + lane;
-    if (i < a.nt) {
+    if (i < nt) {
 #pragma unroll
-      for (int k = 0; k < NO; ++k) out[(size_t)k * a.nt + i] = (double)S[r][k] + (double)C[r][k];
+      for (int k = 0; k < NO; ++k) out[(size_t)k * out_stride + i] = (double)S[r][k] + (double)C[r][k];
     }
+  }
+}
+
+template <class P, typename T, int R, bool COMPENSATED>
+__global__ __launch_bounds__(256) void pair_sum_kernel(PairArgs a) {
+  const int wid = threadIdx.x >> 6;
+  const int nblk_t = (a.n_wtiles + 3) >> 2;
+  const int split = blockIdx.x / nblk_t;
+  const int wtile = (blockIdx.x - split * nblk_t) * 4 + wid;
+  if (wtile >= a.n_wtiles) return;  // wave-uniform
+  const int j_begin = split * a.chunk;
+  const int j_end = min(a.ns, j_begin + a.chunk);
+  pair_tile<P, T, R, COMPENSATED>(reinterpret_cast<const Vec4<T>*>(a.tgt), a.nt, wtile * 64 * R,
+                                  a.tgt_idx0, reinterpret_cast<const T*>(a.src), j_begin, j_end, a.kt,
+                                  a.partial + (size_t)split * P::NOUT * a.nt, a.nt);
+}
+
+// ------------------------------------------------------- batched problems
+// Many independent problems in one persistent launch.  Problem k owns target
+// records [t_off, t_off + nt) (particle indices t_off + local), source records
+// [s_off, s_off + ns) and FP64 partials at part_off ([nsplit][NOUT][nt]).
+// A work item is (problem, wave target tile, source split); items are taken
+// from a device counter by whole wavefronts until the list is exhausted, so
+// every wave reaches the exit.
+struct BatchProblem {
+  int t_off, nt, s_off, ns;
+  int chunk, nsplit;
+  long long part_off;
+};
+
+struct BatchItem {
+  int prob, wtile, split, pad;
+};
+
+struct BatchArgs {
+  const void* tgt;
+  const void* src;
+  double* partial;
+  const BatchProblem* probs;
+  const BatchItem* items;
+  int n_items;
+  int kt;
+  unsigned int* counter;
+};
+
+template <class P, typename T, int R, bool COMPENSATED>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void pair_batch_kernel(BatchArgs b) {
+  const int lane = threadIdx.x & 63;
+  const Vec4<T>* __restrict__ tg = reinterpret_cast<const Vec4<T>*>(b.tgt);
+  const T* __restrict__ src = reinterpret_cast<const T*>(b.src);
+  for (;;) {
+    int item = 0;
+    if (lane == 0) item = (int)atomicAdd(b.counter, 1u);
+    item = __builtin_amdgcn_readfirstlane(__shfl(item, 0));
+    if (item >= b.n_items) break;
+    const BatchItem it = b.items[item];
+    const BatchProblem pr = b.probs[it.prob];
+    const int j_begin = pr.s_off + it.split * pr.chunk;
+    const int j_end = min(pr.s_off + pr.ns, j_begin + pr.chunk);
+    pair_tile<P, T, R, COMPENSATED>(tg + pr.t_off, pr.nt, it.wtile * 64 * R, pr.t_off, src, j_begin,
+                                    j_end, b.kt,
+                                    b.partial + pr.part_off + (size_t)it.split * P::NOUT * pr.nt,
+                                    pr.nt);
   }
 }
 

@@ -79,6 +79,18 @@ _SIGS = {
     "lpm_plane_solver_stream": ([_vp, _pd], _i),
     "lpm_plane_solver_diagnostics": ([_vp, _pd, _pd, _pd], _i),
     "lpm_plane_solver_delete": ([_vp], _i),
+    "lpm_bve_batch_new": (
+        [_vp, ctypes.POINTER(_vp), _i, ctypes.POINTER(ctypes.c_int64), _pd, _pd, _pd, _pd, _pd, _pi, _pd, _pd,
+         _i, _i],
+        _i,
+    ),
+    "lpm_bve_batch_steps": ([_vp, _d, _i], _i),
+    "lpm_bve_batch_get_state": ([_vp, _pd, _pd, _pd, _pd, _pd, _pd, _pd], _i),
+    "lpm_bve_batch_set_tuning": ([_vp, _i, _i, _i], _i),
+    "lpm_bve_batch_get_tuning": ([_vp, _pi, _pi, _pi, _pi], _i),
+    "lpm_bve_batch_autotune": ([_vp, _pi, _pi, _pi, _pd], _i),
+    "lpm_bve_batch_pairs": ([_vp, ctypes.POINTER(ctypes.c_int64)], _i),
+    "lpm_bve_batch_delete": ([_vp], _i),
     "lpm_set_timing": ([_vp, _i], _i),
     "lpm_kernel_stats": ([_vp, _i, ctypes.POINTER(ctypes.c_int64), _pd], _i),
     "lpm_reset_kernel_stats": ([_vp], _i),

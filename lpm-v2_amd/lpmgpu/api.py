@@ -263,3 +263,70 @@ class PlaneSolver:
             self.delete()
         except Exception:
             pass
+
+
+class BVEBatch:
+    """Many independent sphere BVE problems advanced together (lpm_bve_batch_*):
+    one persistent launch per velocity evaluation, optional hipGraph per RK4 stage."""
+
+    def __init__(self, ctx: Context, problems, precision="fp32", use_graph=False, project=False):
+        """problems: list of dicts with x, y, z, relvort, area, active, radius, rotation_rate."""
+        self.ctx = ctx
+        self.sizes = [len(p["x"]) for p in problems]
+        self.offsets = np.concatenate([[0], np.cumsum(self.sizes)]).astype(np.int64)
+        cat = {k: f64(np.concatenate([p[k] for p in problems])) for k in ("x", "y", "z", "relvort", "area")}
+        act = i32(np.concatenate([p["active"] for p in problems]))
+        rad = f64([p.get("radius", 1.0) for p in problems])
+        om = f64([p.get("rotation_rate", 0.0) for p in problems])
+        self.n = int(self.offsets[-1])
+        flags = (L.SOLVER_USE_GRAPH if use_graph else 0) | (L.SOLVER_PROJECT_STAGES if project else 0)
+        h = ctypes.c_void_p()
+        ctx.check(ctx._lib.lpm_bve_batch_new(ctx.handle, ctypes.byref(h), len(problems),
+                                             self.offsets.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)),
+                                             pd(cat["x"]), pd(cat["y"]), pd(cat["z"]), pd(cat["relvort"]),
+                                             pd(cat["area"]), pi(act), pd(rad), pd(om), prec_code(precision), flags))
+        self.handle = h
+
+    def timestep(self, dt: float, nsteps: int = 1):
+        self.ctx.check(self.ctx._lib.lpm_bve_batch_steps(self.handle, dt, nsteps))
+
+    def state(self):
+        out = [np.zeros(self.n) for _ in range(7)]
+        self.ctx.check(self.ctx._lib.lpm_bve_batch_get_state(self.handle, *[pd(o) for o in out]))
+        return dict(zip(["x", "y", "z", "relvort", "u", "v", "w"], out))
+
+    def split(self, state):
+        """Per-problem views of a concatenated state dict."""
+        return [{k: v[a:b] for k, v in state.items()} for a, b in zip(self.offsets[:-1], self.offsets[1:])]
+
+    def set_tuning(self, targets_per_lane, tile_sources, chunk_sources):
+        self.ctx.check(self.ctx._lib.lpm_bve_batch_set_tuning(self.handle, targets_per_lane, tile_sources,
+                                                              chunk_sources))
+
+    def tuning(self):
+        v = [ctypes.c_int32() for _ in range(4)]
+        self.ctx.check(self.ctx._lib.lpm_bve_batch_get_tuning(self.handle, *[ctypes.byref(x) for x in v]))
+        return dict(zip(["targets_per_lane", "tile_sources", "chunk_sources", "n_items"], [x.value for x in v]))
+
+    def autotune(self):
+        v = [ctypes.c_int32() for _ in range(3)]
+        ms = ctypes.c_double()
+        self.ctx.check(self.ctx._lib.lpm_bve_batch_autotune(self.handle, *[ctypes.byref(x) for x in v],
+                                                            ctypes.byref(ms)))
+        return dict(targets_per_lane=v[0].value, tile_sources=v[1].value, chunk_sources=v[2].value, ms=ms.value)
+
+    def pairs_per_eval(self) -> int:
+        p = ctypes.c_int64()
+        self.ctx.check(self.ctx._lib.lpm_bve_batch_pairs(self.handle, ctypes.byref(p)))
+        return p.value
+
+    def delete(self):
+        if getattr(self, "handle", None):
+            self.ctx._lib.lpm_bve_batch_delete(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.delete()
+        except Exception:
+            pass
