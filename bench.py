@@ -88,12 +88,29 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=10.0, help="CPU baseline budget (0 = skip)")
     ap.add_argument("--cpu-threads", type=int, default=int(os.environ.get("OMP_NUM_THREADS", "16")))
     ap.add_argument("--graph", action="store_true", help="replay steps as hipGraphs (no per-kernel timing)")
+    ap.add_argument("--workload", default="n655k",
+                    help="n655k (default, the headline line) or cfg0/cfg1/cfg2/cfg3 for the other workloads")
+    ap.add_argument("--cfg-steps", type=int, default=0, help="override the step count of cfg1/cfg3")
     args = ap.parse_args()
 
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU; the direct sum shards by target blocks "
                           "(see DESIGN.md), --gpus N > 1 is not run"}))
         sys.exit(2)
+
+    if args.workload != "n655k":
+        import lpmgpu
+        from lpmgpu import configs
+
+        ctx = lpmgpu.Context(0)
+        fn = {"cfg0": configs.run_cfg0, "cfg1": configs.run_cfg1, "cfg2": configs.run_cfg2,
+              "cfg3": configs.run_cfg3}[args.workload]
+        kw = {"steps": args.cfg_steps} if args.cfg_steps and args.workload in ("cfg1", "cfg3") else {}
+        res = fn(ctx, log=log, **kw)
+        res["metric"] = METRIC
+        print(json.dumps(res), flush=True)
+        ctx.close()
+        return
 
     import lpmgpu
     from lpmgpu import _lib as L

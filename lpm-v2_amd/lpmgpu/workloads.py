@@ -28,8 +28,8 @@ def cfg1(level=7):
     return dict(mesh=m, relvort=zeta, absvort=zeta + 2 * OMEGA * m.z, dt=0.005, steps=100, omega=OMEGA)
 
 
-def cfg2(n_side=512, seed=BASE_SEED):
-    x, y, area, h = mesh.grid_centres(n_side, 1.0, jitter=1e-3, seed=seed)
+def cfg2(n_side=512, seed=BASE_SEED, jitter=1e-3):
+    x, y, area, h = mesh.grid_centres(n_side, 1.0, jitter=jitter, seed=seed)
     r2 = x * x + y * y
     f = np.exp(-8.0 * r2)
     exact = (256.0 * r2 - 32.0) * np.exp(-8.0 * r2)
