@@ -48,7 +48,8 @@ extern "C" {
 #define LPM_K_PSE2 3         /* PSE Laplacian, 2nd-order kernel */
 #define LPM_K_SPHERE_GREEN 4 /* sphere stream functions         */
 #define LPM_K_PLANE_GREEN 5  /* planar stream function          */
-#define LPM_K_COUNT 6
+#define LPM_K_FAR_FIELD 6    /* learned far field: near + expansion + network */
+#define LPM_K_COUNT 7
 
 typedef struct lpm_ctx lpm_ctx;
 
@@ -211,6 +212,84 @@ int lpm_bve_batch_autotune(lpm_bve_batch *batch, int *targets_per_lane, int *til
 /* Pair interactions of one velocity evaluation (sum over problems of T*S - S). */
 int lpm_bve_batch_pairs(lpm_bve_batch *batch, int64_t *pairs_per_eval);
 int lpm_bve_batch_delete(lpm_bve_batch *batch);
+
+/* -------------------------------------------- far-field surrogate network -- */
+
+/* tanh MLP nin -> width (x nhidden) -> nout, parameters per layer W[out][in]
+ * (row-major) then b[out] in FP32.  Training runs in FP32 on the GPU
+ * (hand-written GEMM forward/backward and Adam); inference runs in BF16 on
+ * MFMA with FP32 accumulation (nin <= 32, width 128, nhidden 3, nout <= 4).
+ * No counterpart in the reference. */
+typedef struct lpm_mlp lpm_mlp;
+
+int lpm_mlp_new(lpm_ctx *ctx, lpm_mlp **mlp, int nin, int width, int nhidden, int nout,
+                uint64_t seed);
+int lpm_mlp_param_count(lpm_mlp *mlp, int64_t *count);
+int lpm_mlp_set_params(lpm_mlp *mlp, const float *params, int64_t count);
+int lpm_mlp_get_params(lpm_mlp *mlp, float *params, int64_t count);
+/* in: [n][nin], out: [n][nout]; use_mfma selects the BF16 MFMA kernel. */
+int lpm_mlp_forward(lpm_mlp *mlp, int n, const float *in, float *out, int use_mfma);
+/* Mean-squared error over n*nout outputs and its gradient (FP32). */
+int lpm_mlp_loss_grad(lpm_mlp *mlp, int n, const float *in, const float *target, float *grad,
+                      double *loss);
+int lpm_mlp_train(lpm_mlp *mlp, int n, const float *in, const float *target, int epochs,
+                  int batch, double lr, uint64_t seed, double *final_loss);
+int lpm_mlp_delete(lpm_mlp *mlp);
+
+/* ------------------------------------------------ learned far field (BVE) -- */
+
+/* Sphere BVE RK4 with the far field of coarse cells replaced by an expansion
+ * (+ the network's correction).  cell[i] in [0, ncell) is particle i's coarse
+ * cell (fixed: cells are Lagrangian).  For target cell t and source cell s at
+ * centroid angle a with angular radii rho_t, rho_s: a < near_radius + rho_t +
+ * rho_s is summed directly (FP32 compensated), everything else by the cell's
+ * second-order expansion in its 10 moments, and cells with a < band_radius +
+ * rho_t + rho_s also get the network's correction of the expansion remainder
+ * (BF16 MFMA).  Stage points are projected onto the sphere (as with
+ * LPM_SOLVER_PROJECT_STAGES).  No counterpart in the reference.
+ * The network (13 -> 128 -> 128 -> 128 -> 3, tanh) sees a cell in the cell's
+ * frame Q (rows e1, e2, e3: e3 along the centroid c, e1 towards the cell's
+ * anchor, the active source farthest from c at set-up): inputs Q (x - c) / l,
+ * M0 / W, Q M1 / (W l), Q M2 Q^T / (W l^2) (xx xy xz yy yz zz), with moments of
+ * w = relvort * area about c, W = sum |w|, l = 2 R sin(rho / 2).  Its output o
+ * gives the remainder of the cell's G / (4 pi R) as Q^T o W / (4 pi R l^2 r^4),
+ * r = |x - c| / l.
+ * mode: 0 plain direct sum (FP32 compensated), 1 near + expansion,
+ *       2 near + expansion + network (needs training or set_params). */
+typedef struct lpm_bve_ff lpm_bve_ff;
+
+int lpm_bve_ff_new(lpm_ctx *ctx, lpm_bve_ff **ff, int n, const double *x, const double *y,
+                   const double *z, const double *relvort, const double *area,
+                   const int32_t *active, const int32_t *cell, int ncell, double radius,
+                   double rotation_rate, double near_radius, double band_radius, int flags);
+/* Velocity of the current state by `mode` into host arrays. */
+int lpm_bve_ff_velocity(lpm_bve_ff *ff, int mode, double *u, double *v, double *w);
+/* Training data from the current state: n_samples (target, band cell) pairs
+ * with the network's 13 inputs and, as target, the exact remainder (direct sum
+ * over the cell minus its expansion, FP64) in the cell frame, times
+ * 4 pi R l^2 r^4 / W; appended to the set. */
+int lpm_bve_ff_collect(lpm_bve_ff *ff, int n_samples, uint64_t seed, int64_t *total);
+/* Train the network on the collected set (Adam with a cosine learning-rate
+ * decay over the epochs, mean-squared error). */
+int lpm_bve_ff_train(lpm_bve_ff *ff, int epochs, int batch, double lr, uint64_t seed,
+                     double *final_loss);
+/* Copy samples [offset, offset + count): in [count][13], target [count][3] and,
+ * when ids is not NULL, ids [count][2] = (target particle, source cell). */
+int lpm_bve_ff_samples(lpm_bve_ff *ff, int64_t offset, int64_t count, float *in, float *target,
+                       int32_t *ids);
+int lpm_bve_ff_set_params(lpm_bve_ff *ff, const float *params, int64_t count);
+int lpm_bve_ff_get_params(lpm_bve_ff *ff, float *params, int64_t count);
+/* nsteps RK4 steps with velocities by `mode`; every judge_every steps the plain
+ * direct sum judges the stored velocity (normwise max relative error) and above
+ * tol the remaining steps fall back to the plain sum.  mode_out = mode in use
+ * at the end, last_error = last judged error (-1 if none), judged = count. */
+int lpm_bve_ff_steps(lpm_bve_ff *ff, double dt, int nsteps, int mode, int judge_every,
+                     double tol, int *mode_out, double *last_error, int *judged);
+int lpm_bve_ff_get_state(lpm_bve_ff *ff, double *x, double *y, double *z, double *relvort,
+                         double *u, double *v, double *w);
+/* Near and band cell counts per target cell (length ncell each). */
+int lpm_bve_ff_cells(lpm_bve_ff *ff, int *near_cells, int *band_cells);
+int lpm_bve_ff_delete(lpm_bve_ff *ff);
 
 /* ----------------------------------------------------- timing / tuning -- */
 

@@ -374,7 +374,7 @@ int collect_timing(lpm_ctx* ctx) {
 int run_pair_sum(lpm_ctx* ctx, const PairProblem& pp) {
   const int kind = pp.kind;
   const int prec = pp.precision;
-  if (kind < 0 || kind >= LPM_K_COUNT) return fail(ctx, LPM_ERR_ARG, "unknown pair kind");
+  if (kind < 0 || kind > LPM_K_PLANE_GREEN) return fail(ctx, LPM_ERR_ARG, "unknown pair kind");
   if (prec != LPM_PREC_FP64 && prec != LPM_PREC_FP32_KAHAN)
     return fail(ctx, LPM_ERR_ARG, "precision must be LPM_PREC_FP64 or LPM_PREC_FP32_KAHAN");
   const int nt = pp.i1 - pp.i0;

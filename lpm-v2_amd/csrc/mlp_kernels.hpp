@@ -1,0 +1,235 @@
+// mlp_kernels.hpp -- the far-field surrogate network on gfx950.
+//
+// Network: nin -> width (tanh) -> ... (nhidden tanh layers) -> nout, parameters
+// per layer W[out][in] (row-major) then b[out], FP32 master copy.
+//
+// Training path (FP32, VALU): tiled GEMMs C = A op(B) with a fused
+// bias + tanh epilogue, the backward GEMMs, column sums and Adam.
+// Inference path (BF16 MFMA, FP32 accumulate), specialised for the
+// 13 -> 128 -> 128 -> 128 -> 3 surrogate: activations are kept transposed
+// (features = MFMA rows, samples = MFMA columns), so the D registers of one
+// layer (lane l: rows 4(l/16)+v of each 16-row tile, column l%16) are, after
+// bias + tanh + BF16 conversion, exactly the B operand of the next layer once
+// the next layer's weight fragments are stored with the matching K order:
+// k-slot e of lane group g in k-step ks is feature 32 ks + 16 (e/4) + 4 g + e%4.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace lpm {
+
+typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
+typedef float floatx4_t __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ float tanh_fast(float x) {
+  // tanh x = 1 - 2 / (exp(2x) + 1), exp via v_exp_f32 (2^y); saturates cleanly
+  const float e = __builtin_amdgcn_exp2f(2.8853900817779268f * x);  // 2 log2(e) x
+  return 1.0f - 2.0f * __builtin_amdgcn_rcpf(e + 1.0f);
+}
+
+// ------------------------------------------------------------ FP32 GEMM
+// C[M][N] = act(alpha * sum_k A(m, k) B(k, n) + bias[n]) (+ beta C), with
+// A(m, k) = TA ? A[k][m] : A[m][k] and B(k, n) = TB ? B[n][k] : B[k][n].
+// 64 x 64 tile per 256-thread block, 4 x 4 outputs per thread, K tiles of 16
+// staged in LDS.  Z (optional) receives the pre-activation.
+struct GemmArgs {
+  const float* A;
+  const float* B;
+  float* C;
+  float* Z;
+  const float* bias;
+  int M, N, K;
+  int lda, ldb, ldc;
+  float alpha, beta;
+  int act;  // 0 none, 1 tanh
+};
+
+template <bool TA, bool TB>
+__global__ __launch_bounds__(256) void gemm_f32(GemmArgs g) {
+  __shared__ float As[16][64 + 1];
+  __shared__ float Bs[16][64 + 1];
+  const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
+  const int m0 = blockIdx.y * 64, n0 = blockIdx.x * 64;
+  float acc[4][4] = {};
+  for (int k0 = 0; k0 < g.K; k0 += 16) {
+    for (int t = threadIdx.x; t < 16 * 64; t += 256) {
+      const int kk = t / 64, mm = t % 64;
+      const int m = m0 + mm, k = k0 + kk;
+      float a = 0.f;
+      if (m < g.M && k < g.K) a = TA ? g.A[(size_t)k * g.lda + m] : g.A[(size_t)m * g.lda + k];
+      As[kk][mm] = a;
+      const int n = n0 + mm;
+      float b = 0.f;
+      if (n < g.N && k < g.K) b = TB ? g.B[(size_t)n * g.ldb + k] : g.B[(size_t)k * g.ldb + n];
+      Bs[kk][mm] = b;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int kk = 0; kk < 16; ++kk) {
+      float a[4], b[4];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) a[i] = As[kk][ty * 4 + i];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) b[j] = Bs[kk][tx * 4 + j];
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc[i][j] = fmaf(a[i], b[j], acc[i][j]);
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const int m = m0 + ty * 4 + i;
+    if (m >= g.M) continue;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int n = n0 + tx * 4 + j;
+      if (n >= g.N) continue;
+      float v = g.alpha * acc[i][j];
+      if (g.bias) v += g.bias[n];
+      if (g.beta != 0.f) v += g.beta * g.C[(size_t)m * g.ldc + n];
+      if (g.Z) g.Z[(size_t)m * g.ldc + n] = v;
+      if (g.act == 1) v = tanhf(v);
+      g.C[(size_t)m * g.ldc + n] = v;
+    }
+  }
+}
+
+// ------------------------------------------------------- BF16 MFMA forward
+// Weight fragments, staged once per block in LDS:
+//   W1: [8 rt][64 lanes][8]      (K = 32, inputs 0..nin-1, zero padded)
+//   W2, W3: [8 rt][4 ks][64][8]  (K order permuted as described above)
+//   W4: [4 ks][64][8]            (rows 0..nout-1 of one 16-row tile)
+// bias: b1, b2, b3 (128 each), b4 (16), FP32.
+constexpr int kW1Frag = 8 * 64 * 8;
+constexpr int kWHFrag = 8 * 4 * 64 * 8;
+constexpr int kW4Frag = 4 * 64 * 8;
+constexpr int kMlpFragElems = kW1Frag + 2 * kWHFrag + kW4Frag;  // bf16 elements
+constexpr int kMlpBiasElems = 128 * 3 + 16;
+
+__host__ __device__ inline int hidden_k_feature(int ks, int g, int e) {
+  return 32 * ks + 16 * (e / 4) + 4 * g + (e % 4);
+}
+
+// Convert FP32 parameters (13 -> 128 x3 -> 3 layout) into fragment order.
+static __global__ __launch_bounds__(256) void mlp_pack_fragments(const float* __restrict__ P, int nin, int nout,
+                                                         __bf16* __restrict__ frag,
+                                                         float* __restrict__ bias) {
+  const int W = 128;
+  const float* W1 = P;
+  const float* b1 = W1 + W * nin;
+  const float* W2 = b1 + W;
+  const float* b2 = W2 + W * W;
+  const float* W3 = b2 + W;
+  const float* b3 = W3 + W * W;
+  const float* W4 = b3 + W;
+  const float* b4 = W4 + nout * W;
+  for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < kMlpFragElems; t += gridDim.x * blockDim.x) {
+    float v = 0.f;
+    if (t < kW1Frag) {
+      const int rt = t / 512, l = (t / 8) % 64, e = t % 8;
+      const int row = 16 * rt + l % 16, k = 8 * (l / 16) + e;
+      v = k < nin ? W1[row * nin + k] : 0.f;
+    } else if (t < kW1Frag + 2 * kWHFrag) {
+      const int u = (t - kW1Frag) % kWHFrag;
+      const float* Wm = (t - kW1Frag) < kWHFrag ? W2 : W3;
+      const int rt = u / 2048, ks = (u / 512) % 4, l = (u / 8) % 64, e = u % 8;
+      v = Wm[(16 * rt + l % 16) * W + hidden_k_feature(ks, l / 16, e)];
+    } else {
+      const int u = t - kW1Frag - 2 * kWHFrag;
+      const int ks = u / 512, l = (u / 8) % 64, e = u % 8;
+      const int row = l % 16;
+      v = row < nout ? W4[row * W + hidden_k_feature(ks, l / 16, e)] : 0.f;
+    }
+    frag[t] = (__bf16)v;
+  }
+  for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < kMlpBiasElems; t += gridDim.x * blockDim.x) {
+    float v;
+    if (t < 128) v = b1[t];
+    else if (t < 256) v = b2[t - 128];
+    else if (t < 384) v = b3[t - 256];
+    else v = (t - 384) < nout ? b4[t - 384] : 0.f;
+    bias[t] = v;
+  }
+}
+
+// One wavefront evaluates NCT x 16 samples.  in8[ct] holds this lane's 8 input
+// features (features 8 g .. 8 g + 7 of sample 16 ct + l % 16, g = l / 16);
+// on return out[ct][0..2] are the outputs of that sample in lanes 0..15
+// (lanes 16..63 hold padding rows).  Weights/biases in LDS (fragment order).
+template <int NCT>
+__device__ __forceinline__ void mlp_wave_forward(const bf16x8_t in8[NCT], const __bf16* __restrict__ sfrag,
+                                                 const float* __restrict__ sbias, float out[NCT][4]) {
+  const int lane = threadIdx.x & 63;
+  const int g = lane >> 4;
+  const bf16x8_t* F1 = reinterpret_cast<const bf16x8_t*>(sfrag);
+  const bf16x8_t* F2 = reinterpret_cast<const bf16x8_t*>(sfrag + kW1Frag);
+  const bf16x8_t* F3 = reinterpret_cast<const bf16x8_t*>(sfrag + kW1Frag + kWHFrag);
+  const bf16x8_t* F4 = reinterpret_cast<const bf16x8_t*>(sfrag + kW1Frag + 2 * kWHFrag);
+  floatx4_t acc[8][NCT];
+  bf16x8_t h[4][NCT];  // B operands of the next layer: [ks][ct]
+
+  // layer 1: K = 32 (one step)
+#pragma unroll
+  for (int rt = 0; rt < 8; ++rt) {
+    const bf16x8_t a = F1[rt * 64 + lane];
+#pragma unroll
+    for (int ct = 0; ct < NCT; ++ct) {
+      floatx4_t z = {0.f, 0.f, 0.f, 0.f};
+      acc[rt][ct] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, in8[ct], z, 0, 0, 0);
+    }
+  }
+  // bias + tanh -> next B operands
+  auto activate = [&](const float* b) {
+#pragma unroll
+    for (int ks = 0; ks < 4; ++ks)
+#pragma unroll
+      for (int ct = 0; ct < NCT; ++ct) {
+        bf16x8_t v;
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+          const int rt = 2 * ks + e / 4, vv = e % 4;
+          const float z = acc[rt][ct][vv] + b[16 * rt + 4 * g + vv];
+          v[e] = (__bf16)tanh_fast(z);
+        }
+        h[ks][ct] = v;
+      }
+  };
+  activate(sbias);
+  // layers 2, 3: 128 x 128
+#pragma unroll 1
+  for (int layer = 0; layer < 2; ++layer) {
+    const bf16x8_t* F = layer == 0 ? F2 : F3;
+#pragma unroll
+    for (int rt = 0; rt < 8; ++rt) {
+#pragma unroll
+      for (int ct = 0; ct < NCT; ++ct) acc[rt][ct] = (floatx4_t){0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+      for (int ks = 0; ks < 4; ++ks) {
+        const bf16x8_t a = F[(rt * 4 + ks) * 64 + lane];
+#pragma unroll
+        for (int ct = 0; ct < NCT; ++ct)
+          acc[rt][ct] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, h[ks][ct], acc[rt][ct], 0, 0, 0);
+      }
+    }
+    activate(sbias + 128 * (layer + 1));
+  }
+  // output layer: one 16-row tile, K = 128
+  floatx4_t o[NCT];
+#pragma unroll
+  for (int ct = 0; ct < NCT; ++ct) o[ct] = (floatx4_t){0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+  for (int ks = 0; ks < 4; ++ks) {
+    const bf16x8_t a = F4[ks * 64 + lane];
+#pragma unroll
+    for (int ct = 0; ct < NCT; ++ct) o[ct] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, h[ks][ct], o[ct], 0, 0, 0);
+  }
+#pragma unroll
+  for (int ct = 0; ct < NCT; ++ct)
+#pragma unroll
+    for (int v = 0; v < 4; ++v) out[ct][v] = o[ct][v] + sbias[384 + 4 * g + v];
+}
+
+}  // namespace lpm

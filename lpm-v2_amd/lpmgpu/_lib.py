@@ -91,6 +91,16 @@ _SIGS = {
     "lpm_bve_batch_autotune": ([_vp, _pi, _pi, _pi, _pd], _i),
     "lpm_bve_batch_pairs": ([_vp, ctypes.POINTER(ctypes.c_int64)], _i),
     "lpm_bve_batch_delete": ([_vp], _i),
+    "lpm_mlp_new": ([_vp, ctypes.POINTER(_vp), _i, _i, _i, _i, ctypes.c_uint64], _i),
+    "lpm_mlp_param_count": ([_vp, ctypes.POINTER(ctypes.c_int64)], _i),
+    "lpm_mlp_set_params": ([_vp, ctypes.POINTER(ctypes.c_float), ctypes.c_int64], _i),
+    "lpm_mlp_get_params": ([_vp, ctypes.POINTER(ctypes.c_float), ctypes.c_int64], _i),
+    "lpm_mlp_forward": ([_vp, _i, ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float), _i], _i),
+    "lpm_mlp_loss_grad": ([_vp, _i, ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float),
+                           ctypes.POINTER(ctypes.c_float), _pd], _i),
+    "lpm_mlp_train": ([_vp, _i, ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float), _i, _i, _d,
+                       ctypes.c_uint64, _pd], _i),
+    "lpm_mlp_delete": ([_vp], _i),
     "lpm_set_timing": ([_vp, _i], _i),
     "lpm_kernel_stats": ([_vp, _i, ctypes.POINTER(ctypes.c_int64), _pd], _i),
     "lpm_reset_kernel_stats": ([_vp], _i),

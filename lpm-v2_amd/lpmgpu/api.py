@@ -11,6 +11,7 @@ whose entries outside the block are zero.
 from __future__ import annotations
 
 import ctypes
+import weakref
 
 import numpy as np
 
@@ -29,14 +30,23 @@ class Context:
             raise LpmError(rc, self._lib.lpm_last_error(None).decode())
         self.handle = h
         self.device = device
+        # objects created on this context (solvers, batches, networks): they
+        # hold pointers into it and are deleted before the context is
+        self._children = weakref.WeakSet()
 
     # -- error plumbing
     def check(self, rc: int):
         if rc != L.LPM_OK:
             raise LpmError(rc, self._lib.lpm_last_error(self.handle).decode())
 
+    def adopt(self, obj):
+        self._children.add(obj)
+        return obj
+
     def close(self):
         if getattr(self, "handle", None):
+            for child in list(self._children):
+                child.delete()
             self._lib.lpm_destroy(self.handle)
             self.handle = None
 
@@ -185,6 +195,7 @@ class BVESolver:
                                               pd(arrs[2]), pd(arrs[3]), pd(absv), pd(arrs[4]), pi(act), pd(uu),
                                               pd(vv), pd(ww), radius, rotation_rate, prec_code(precision), flags))
         self.handle = h
+        ctx.adopt(self)
 
     def timestep(self, dt: float, nsteps: int = 1):
         self.ctx.check(self.ctx._lib.lpm_bve_solver_steps(self.handle, dt, nsteps))
@@ -206,7 +217,7 @@ class BVESolver:
         return tuple(o.value for o in out)
 
     def delete(self):
-        if getattr(self, "handle", None):
+        if getattr(self, "handle", None) and self.ctx.handle:
             self.ctx._lib.lpm_bve_solver_delete(self.handle)
             self.handle = None
 
@@ -234,6 +245,7 @@ class PlaneSolver:
                                                 pd(arrs[2]), pd(arrs[3]), pi(act), pd(uu), pd(vv),
                                                 prec_code(precision), flags))
         self.handle = h
+        ctx.adopt(self)
 
     def timestep(self, dt: float, nsteps: int = 1):
         self.ctx.check(self.ctx._lib.lpm_plane_solver_steps(self.handle, dt, nsteps))
@@ -254,7 +266,7 @@ class PlaneSolver:
         return tuple(o.value for o in out)
 
     def delete(self):
-        if getattr(self, "handle", None):
+        if getattr(self, "handle", None) and self.ctx.handle:
             self.ctx._lib.lpm_plane_solver_delete(self.handle)
             self.handle = None
 
@@ -286,6 +298,7 @@ class BVEBatch:
                                              pd(cat["x"]), pd(cat["y"]), pd(cat["z"]), pd(cat["relvort"]),
                                              pd(cat["area"]), pi(act), pd(rad), pd(om), prec_code(precision), flags))
         self.handle = h
+        ctx.adopt(self)
 
     def timestep(self, dt: float, nsteps: int = 1):
         self.ctx.check(self.ctx._lib.lpm_bve_batch_steps(self.handle, dt, nsteps))
@@ -321,7 +334,7 @@ class BVEBatch:
         return p.value
 
     def delete(self):
-        if getattr(self, "handle", None):
+        if getattr(self, "handle", None) and self.ctx.handle:
             self.ctx._lib.lpm_bve_batch_delete(self.handle)
             self.handle = None
 

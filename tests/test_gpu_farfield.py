@@ -1,0 +1,136 @@
+"""Learned far field on the GPU: the near/far split, the cell expansion, the
+training samples and the judge/fallback, against the plain sum and an
+independent finite-difference Taylor expansion in numpy."""
+import numpy as np
+import pytest
+
+from helpers import rel_err_max
+from lpmgpu import ic, mesh
+from lpmgpu.farfield import EXPANSION, LEARNED, PLAIN, FarFieldBVE
+
+pytestmark = pytest.mark.gpu
+
+
+def _case(level=5, cell_level=3):
+    m = mesh.icosahedral_mesh(level, cell_level=cell_level)
+    zeta = ic.gaussian_vortex(m.x, m.y, m.z, ic.lat_lon_point(0.3, 0.0), 4 * np.pi, 16.0) + ic.rh4_vorticity(
+        m.x, m.y, m.z)
+    return m, zeta
+
+
+def test_all_near_equals_plain(ctx):
+    """near_radius = pi: every cell is summed directly (self pairs excluded
+    exactly), so the result is the plain sum up to FP32 summation order."""
+    m, zeta = _case(4, 2)
+    f = FarFieldBVE(ctx, m, zeta, near_radius=np.pi, band_radius=np.pi)
+    p = f.velocity(PLAIN)
+    e = f.velocity(EXPANSION)
+    print("all-near vs plain", rel_err_max(e, p))
+    assert rel_err_max(e, p) < 1e-6
+    f.delete()
+
+
+def test_expansion_error_falls_with_cutoff(ctx):
+    m, zeta = _case(5, 3)
+    errs = []
+    for r in (0.05, 0.15, 0.3, 0.6):
+        f = FarFieldBVE(ctx, m, zeta, near_radius=r, band_radius=r)
+        errs.append(rel_err_max(f.velocity(EXPANSION), f.velocity(PLAIN)))
+        f.delete()
+    print("expansion error vs near radius", errs)
+    assert errs[-1] < 1e-4 and all(a > b for a, b in zip(errs, errs[1:]))
+
+
+def _cell_frame(P, idx, area):
+    """Area-weighted centroid, frame rows (e1 towards the farthest source, e3
+    along c) and whether that farthest source is unambiguous."""
+    c = (P[idx] * area[idx, None]).sum(0) / area[idx].sum()
+    d2 = ((P[idx] - c) ** 2).sum(1)
+    top = np.sort(d2)[::-1]
+    clear = len(top) < 2 or top[0] - top[1] > 1e-9 * top[0]
+    a = P[idx[np.argmax(d2)]] - c
+    e3 = c / np.linalg.norm(c)
+    e1 = a - (a @ e3) * e3
+    e1 /= np.linalg.norm(e1)
+    return c, np.stack([e1, np.cross(e3, e1), e3]), clear
+
+
+def test_training_samples_match_numpy(ctx):
+    """A sample's inputs are the target and the cell's moments in the cell frame;
+    its target is Q (exact cell sum - 2nd-order expansion) l^2 r^4 / W.  Check
+    both with numpy: exact sum over the cell, expansion from a finite-difference
+    Taylor series of y / (R^2 - x.y) about the centroid (not the closed form)."""
+    m, zeta = _case(5, 3)
+    f = FarFieldBVE(ctx, m, zeta, near_radius=0.1, band_radius=0.45)
+    f.collect(3000, seed=3)
+    X, T, ids = f.samples(0, 3000)
+    assert np.all(np.isfinite(X)) and np.all(np.isfinite(T))
+    assert (ids[:, 0] >= 0).mean() > 0.99
+    cells = f.cell
+    act = m.active == 1
+    w = zeta * m.area
+    P = m.xyz()
+    checked = 0
+    for q in range(0, 3000, 23):
+        p, s = ids[q]
+        if p < 0:
+            continue
+        idx = np.flatnonzero(act & (cells == s))
+        c, Q, clear = _cell_frame(P, idx, m.area)
+        if not clear:  # symmetric cell: the anchor is a tie broken by rounding
+            continue
+        W = np.abs(w[idx]).sum()
+        rho = np.max(np.arccos(np.clip(P[idx] @ Q[2], -1, 1)))
+        ell = 2 * np.sin(0.5 * rho)
+        x = P[p]
+        d = P[idx] - c
+        M0 = w[idx].sum()
+        M1 = (w[idx, None] * d).sum(0)
+        M2 = np.einsum("j,ja,jb->ab", w[idx], d, d)
+        loc = Q @ (x - c) / ell
+        M2l = Q @ M2 @ Q.T / (W * ell * ell)
+        feat = np.concatenate([loc, [M0 / W], Q @ M1 / (W * ell), M2l[[0, 0, 0, 1, 1, 2], [0, 1, 2, 1, 2, 2]]])
+        assert np.allclose(X[q], feat, rtol=1e-5, atol=1e-5 * np.abs(feat).max()), (X[q], feat)
+
+        def g(y):
+            return y / (1.0 - y @ x)
+
+        exact = (w[idx, None] * P[idx] / (1.0 - P[idx] @ x)[:, None]).sum(0)
+        # |G| is up to ~1e3 x the remainder near the vortex: the differences need
+        # h = 1e-5 (truncation ~(h / D)^2 |G|, rounding ~eps |G| / h^2)
+        h = 1e-5
+        E = np.eye(3)
+        jac = np.stack([(g(c + h * E[a]) - g(c - h * E[a])) / (2 * h) for a in range(3)], 0)
+        hes = np.zeros((3, 3, 3))
+        for a in range(3):
+            for b in range(3):
+                hes[a, b] = (g(c + h * E[a] + h * E[b]) - g(c + h * E[a] - h * E[b]) - g(c - h * E[a] + h * E[b])
+                             + g(c - h * E[a] - h * E[b])) / (4 * h * h)
+        expn = M0 * g(c) + M1 @ jac + 0.5 * np.einsum("ab,abk->k", M2, hes)
+        want = Q @ (exact - expn) * ell ** 2 * (loc @ loc) ** 2 / W
+        assert np.allclose(T[q], want, rtol=1e-3, atol=1e-4 * np.abs(want).max() + 1e-7), (T[q], want)
+        checked += 1
+    assert checked >= 60
+    f.delete()
+
+
+def test_network_gain_and_judge(ctx):
+    """Train on samples from the method's own state, then compare the error of
+    near + expansion (the method without the network) with near + expansion +
+    network at the same cutoff; the judge falls back above tol."""
+    m, zeta = _case(5, 3)
+    f = FarFieldBVE(ctx, m, zeta, rotation_rate=2 * np.pi, near_radius=0.1, band_radius=0.45)
+    f.collect(400000, seed=1)
+    loss = f.train(epochs=20, batch=1024, lr=1e-3, seed=2)
+    plain = f.velocity(PLAIN)
+    e1 = rel_err_max(f.velocity(EXPANSION), plain)
+    e2 = rel_err_max(f.velocity(LEARNED), plain)
+    print(f"train loss {loss:.3e}; error without network {e1:.3e}, with network {e2:.3e}, gain {e1 / e2:.2f}x")
+    assert np.isfinite(e2) and e2 < 0.5 * e1
+    r = f.steps(0.01, 4, LEARNED, judge_every=2, tol=1e-12)
+    assert r["mode"] == PLAIN and r["judged"] == 1 and r["last_error"] > 1e-12
+    r = f.steps(0.01, 4, EXPANSION, judge_every=2, tol=1.0)
+    assert r["mode"] == EXPANSION and r["judged"] == 2 and 0 <= r["last_error"] < 1.0
+    st = f.state()
+    assert np.abs(np.sqrt(st["x"] ** 2 + st["y"] ** 2 + st["z"] ** 2) - 1).max() < 1e-14
+    f.delete()

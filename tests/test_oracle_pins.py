@@ -405,3 +405,66 @@ def test_pse_eps():
 
     assert O.pse_eps(0.01, 0.75) == pytest.approx(0.01**0.75, rel=1e-15)
     assert pse_eps(0.01, 0.75) == pytest.approx(0.01**0.75, rel=1e-15)
+
+
+# ---------------------------------------------------- far-field surrogate pins
+def test_mlp_oracle_gradient_matches_finite_differences():
+    """The FP64 MLP oracle's analytic gradient against central differences
+    (both activations), on a small network and batch."""
+    rng = np.random.default_rng(21)
+    nin, width, nh, nout = 5, 7, 3, 2
+    npar = O.mlp_param_count(nin, width, nh, nout)
+    p = rng.normal(scale=0.5, size=npar)
+    x = rng.normal(size=(6, nin))
+    t = rng.normal(size=(6, nout))
+    for act in (0, 1):
+        loss, g = O.mlp_loss_grad(p, x, t, nin, width, nh, nout, act)
+        fd = np.zeros(npar)
+        h = 1e-6
+        for k in range(npar):
+            pp = p.copy()
+            pp[k] += h
+            lp, _ = O.mlp_loss_grad(pp, x, t, nin, width, nh, nout, act)
+            pp[k] -= 2 * h
+            lm, _ = O.mlp_loss_grad(pp, x, t, nin, width, nh, nout, act)
+            fd[k] = (lp - lm) / (2 * h)
+        tol = 1e-6 if act == 1 else 1e-4  # ReLU kinks
+        assert np.abs(fd - g).max() < tol * max(1.0, np.abs(g).max()), act
+
+
+def test_mlp_oracle_forward_by_hand():
+    """A 1-hidden-layer tanh network written out by hand."""
+    nin, width, nout = 2, 3, 1
+    W1 = np.array([[0.5, -1.0], [2.0, 0.25], [0.0, 1.0]])
+    b1 = np.array([0.1, -0.2, 0.3])
+    W2 = np.array([[1.0, -2.0, 0.5]])
+    b2 = np.array([0.7])
+    p = np.concatenate([W1.ravel(), b1, W2.ravel(), b2])
+    x = np.array([[0.3, -0.4], [1.0, 2.0]])
+    want = np.tanh(x @ W1.T + b1) @ W2.T + b2
+    got = O.mlp_forward(p, x, nin, width, 1, nout, 1)
+    assert np.allclose(got, want, rtol=1e-14, atol=1e-14)
+
+
+def test_cell_moments_parallel_axis():
+    """Moments about two centroids obey the parallel-axis relations
+    M1' = M1 - M0 s, M2' = M2 - s M1^T - M1 s^T + M0 s s^T (s = c' - c)."""
+    rng = np.random.default_rng(22)
+    n, ncell = 200, 3
+    x, y, z = rng.normal(size=(3, n))
+    w = rng.normal(size=n)
+    cell = rng.integers(0, ncell, n)
+    c = rng.normal(size=(ncell, 3))
+    s = rng.normal(size=(ncell, 3)) * 0.3
+    M = O.cell_moments(x, y, z, w, cell, ncell, c)
+    Mp = O.cell_moments(x, y, z, w, cell, ncell, c + s)
+    iu = [(0, 0), (0, 1), (0, 2), (1, 1), (1, 2), (2, 2)]
+    for k in range(ncell):
+        M0, M1 = M[k, 0], M[k, 1:4]
+        M2 = np.zeros((3, 3))
+        for q, (a, b) in enumerate(iu):
+            M2[a, b] = M2[b, a] = M[k, 4 + q]
+        assert np.isclose(Mp[k, 0], M0)
+        assert np.allclose(Mp[k, 1:4], M1 - M0 * s[k], atol=1e-12)
+        M2p = M2 - np.outer(s[k], M1) - np.outer(M1, s[k]) + M0 * np.outer(s[k], s[k])
+        assert np.allclose([Mp[k, 4 + q] for q in range(6)], [M2p[a, b] for a, b in iu], atol=1e-12)
