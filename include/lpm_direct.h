@@ -240,23 +240,30 @@ int lpm_mlp_delete(lpm_mlp *mlp);
 
 /* Sphere BVE RK4 with the far field of coarse cells replaced by an expansion
  * (+ the network's correction).  cell[i] in [0, ncell) is particle i's coarse
- * cell (fixed: cells are Lagrangian).  For target cell t and source cell s at
+ * cell at set-up; it defines each cell's generator (the direction of the
+ * area-weighted centroid of its active sources) and frame anchor, and every
+ * evaluation bins the particles to the nearest generator, so cells stay fixed
+ * regions of the sphere.  For target cell t and source cell s at
  * centroid angle a with angular radii rho_t, rho_s: a < near_radius + rho_t +
  * rho_s is summed directly (FP32 compensated), everything else by the cell's
- * second-order expansion in its 10 moments, and cells with a < band_radius +
- * rho_t + rho_s also get the network's correction of the expansion remainder
- * (BF16 MFMA).  Stage points are projected onto the sphere (as with
- * LPM_SOLVER_PROJECT_STAGES).  No counterpart in the reference.
+ * expansion of order P in its moments M0..MP about the centroid (P = 2: 10
+ * moments; P = 4: 35), and cells with a < band_radius + rho_t + rho_s also get
+ * the network's correction of the expansion remainder (BF16 MFMA).  Stage
+ * points are projected onto the sphere (as with LPM_SOLVER_PROJECT_STAGES).
+ * flags: LPM_FF_ORDER(P) with P in 2..4 (0 selects 4).  No counterpart in the
+ * reference.
  * The network (13 -> 128 -> 128 -> 128 -> 3, tanh) sees a cell in the cell's
  * frame Q (rows e1, e2, e3: e3 along the centroid c, e1 towards the cell's
  * anchor, the active source farthest from c at set-up): inputs Q (x - c) / l,
  * M0 / W, Q M1 / (W l), Q M2 Q^T / (W l^2) (xx xy xz yy yz zz), with moments of
  * w = relvort * area about c, W = sum |w|, l = 2 R sin(rho / 2).  Its output o
- * gives the remainder of the cell's G / (4 pi R) as Q^T o W / (4 pi R l^2 r^4),
+ * gives the remainder of the cell's G / (4 pi R) as Q^T o W / (4 pi R l^2 r^(P+2)),
  * r = |x - c| / l.
  * mode: 0 plain direct sum (FP32 compensated), 1 near + expansion,
  *       2 near + expansion + network (needs training or set_params). */
 typedef struct lpm_bve_ff lpm_bve_ff;
+#define LPM_FF_ORDER_MASK 7
+#define LPM_FF_ORDER(p) ((p) & LPM_FF_ORDER_MASK)
 
 int lpm_bve_ff_new(lpm_ctx *ctx, lpm_bve_ff **ff, int n, const double *x, const double *y,
                    const double *z, const double *relvort, const double *area,
@@ -267,12 +274,17 @@ int lpm_bve_ff_velocity(lpm_bve_ff *ff, int mode, double *u, double *v, double *
 /* Training data from the current state: n_samples (target, band cell) pairs
  * with the network's 13 inputs and, as target, the exact remainder (direct sum
  * over the cell minus its expansion, FP64) in the cell frame, times
- * 4 pi R l^2 r^4 / W; appended to the set. */
+ * 4 pi R l^2 r^(P+2) / W; appended to the set. */
 int lpm_bve_ff_collect(lpm_bve_ff *ff, int n_samples, uint64_t seed, int64_t *total);
 /* Train the network on the collected set (Adam with a cosine learning-rate
- * decay over the epochs, mean-squared error). */
+ * decay over the epochs, mean-squared error of the targets divided by their
+ * RMS, which the first training fixes and folds into the output layer). */
 int lpm_bve_ff_train(lpm_bve_ff *ff, int epochs, int batch, double lr, uint64_t seed,
                      double *final_loss);
+/* Online fine-tuning in lpm_bve_ff_steps: at every judge that keeps the learned
+ * mode, replace the collected set by n_samples samples of the judged state and
+ * train `epochs` epochs on it.  n_samples = 0 (the default) turns it off. */
+int lpm_bve_ff_set_online(lpm_bve_ff *ff, int n_samples, int epochs, int batch, double lr);
 /* Copy samples [offset, offset + count): in [count][13], target [count][3] and,
  * when ids is not NULL, ids [count][2] = (target particle, source cell). */
 int lpm_bve_ff_samples(lpm_bve_ff *ff, int64_t offset, int64_t count, float *in, float *target,
@@ -287,6 +299,13 @@ int lpm_bve_ff_steps(lpm_bve_ff *ff, double dt, int nsteps, int mode, int judge_
                      double tol, int *mode_out, double *last_error, int *judged);
 int lpm_bve_ff_get_state(lpm_bve_ff *ff, double *x, double *y, double *z, double *relvort,
                          double *u, double *v, double *w);
+/* Replace positions (projected onto the sphere) and relative vorticity, and
+ * store the plain velocity of the new state; cells, anchors, set-up radii, the
+ * network and the collected samples are kept. */
+int lpm_bve_ff_set_state(lpm_bve_ff *ff, const double *x, const double *y, const double *z,
+                         const double *relvort);
+/* Cell of every particle (length n) as binned by the last evaluation. */
+int lpm_bve_ff_particle_cells(lpm_bve_ff *ff, int32_t *cell);
 /* Near and band cell counts per target cell (length ncell each). */
 int lpm_bve_ff_cells(lpm_bve_ff *ff, int *near_cells, int *band_cells);
 int lpm_bve_ff_delete(lpm_bve_ff *ff);

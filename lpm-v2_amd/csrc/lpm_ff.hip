@@ -4,31 +4,40 @@
 // For each target cell t and source cell s, with cell centroids c, angular
 // radii rho and the angle a(t, s) between their centroid directions:
 //   near   a < r_near + rho_t + rho_s : direct FP32-compensated sum over s's sources
-//   far    otherwise                  : analytic expansion of s to 2nd order in
-//          its 10 moments (M0, M1, M2 of zeta A about c_s)
+//   far    otherwise                  : analytic expansion of s to order P (2..4)
+//          in its moments M0..MP of w = zeta A about c_s (P = 4: 35 numbers)
 //   band   far and a < r_band + rho_t + rho_s : far + the MLP's correction of the
 //          expansion's remainder (BF16 MFMA), trained on direct sums
 // Cell membership is Lagrangian (fixed at set-up); centroids, radii and
 // moments follow the particles and are recomputed at every evaluation.
 //
-// The network works in a frame of its cell: e3 along the centroid c, e1 towards
-// the cell's anchor source (the source farthest from c at set-up, fixed), e2 =
-// e3 x e1.  Cells of one mesh level then look alike to it, which fixes the
-// orientation of the cell's third moment that the 10 moments leave open.
-// Inputs (13): Q (x - c) / l, M0 / W, Q M1 / (W l), Q M2 Q^T / (W l^2) (6), with
-// Q the frame rows, l = 2 R sin(rho / 2) and W = sum |w|.  Output o (3, cell
-// frame): the expansion's remainder of G_s / (4 pi R) is Q^T o s_out / r^4 with
-// r = |x - c| / l and s_out = W / (4 pi R l^2), a scaling under which the
-// remainder (~ W l^3 / |x - c|^5) is O(1) at every distance in the band.
-//
 // Velocity assembly, per target x (on the sphere of radius R):
 //   u = V x x,  V = sum_near s_ij (x - y_j) + sum_far G_s(x) / (4 pi R)
-// with s_ij = -zeta_j A_j / (4 pi R (R^2 - x.y_j)) and, for the expansion,
-// G_s(x) = c alpha + M1 / D + (M2 x) / D^2,
-// alpha = M0 / D + x.M1 / D^2 + x^T M2 x / D^3, D = R^2 - x.c
-// (sum_j w_j y_j / (R^2 - x.y_j) to second order about c; the parts of V
-// along x drop out of V x x).  D is evaluated as (|x-c|^2 + R^2 - |c|^2) / 2.
+// with s_ij = -zeta_j A_j / (4 pi R (R^2 - x.y_j)); the parts of V along x drop
+// out of V x x.  G_s(x) = sum_j w_j y_j / (R^2 - x.y_j) expanded about c: with
+// phi = 1 / D, D = R^2 - x.c (evaluated as (|x-c|^2 + R^2 - |c|^2) / 2) and
+// 1 / (D - x.d) = phi sum_k (phi x.d)^k for y = c + d,
+//   G_s = c alpha + beta,  alpha = sum_{k=0..P} phi^(k+1) M_k[x,..,x],
+//                          beta  = sum_{k=1..P} phi^k M_k[., x,..,x]
+// (M_k[., x^(k-1)] the k-th moment tensor contracted k-1 times with x).
+//
+// Cells are fixed regions: a particle belongs to the cell whose generator (the
+// set-up centroid direction of a coarse mesh face) is nearest, and particles
+// are binned anew (stable radix sort by cell) at every evaluation, so cells
+// keep their size however the flow strains the particles.
+//
+// The network works in a frame of its cell: e3 along the centroid c, e1 towards
+// the cell's anchor direction (fixed at set-up: its source farthest from c),
+// e2 = e3 x e1.  Cells of one mesh level then look alike to it, which fixes the
+// orientation of the cell's odd moments that the 10 moments leave open.
+// Inputs (13): Q (x - c) / l, M0 / W, Q M1 / (W l), Q M2 Q^T / (W l^2) (6), with
+// Q the frame rows, l = 2 R sin(rho / 2) and W = sum |w|.  Output o (3, cell
+// frame): the expansion's remainder of G_s / (4 pi R) is Q^T o s_out / r^(P+2)
+// with r = |x - c| / l and s_out = W / (4 pi R l^2), a scaling under which the
+// remainder (~ W l^(P+1) / |x - c|^(P+3)) is O(1) at every distance in the band.
 #include <math.h>
+
+#include <hipcub/hipcub.hpp>
 
 #include <algorithm>
 #include <numeric>
@@ -49,9 +58,124 @@ namespace {
 constexpr double kPi = 3.14159265358979323846;
 constexpr int kR = 4;        // targets per lane in the near / far kernels
 constexpr int kKT = 256;     // sources per compensated tile
-constexpr int kCellRec = 16; // floats per far-field cell record
+constexpr int kNMom = 35;    // moments M0..M4: 1 + 3 + 6 + 10 + 15
+constexpr int kCellRec = 40; // floats per far-field cell record
 constexpr int kMlpRec = 32;  // floats per MLP cell record
-constexpr int kOutPow = 4;   // remainder scaling power r^kOutPow
+
+// Symmetric moment tensors are stored by their sorted index tuples in
+// lexicographic order: M2 {00 01 02 11 12 22}, M3 {000 001 002 011 012 022 111
+// 112 122 222}, M4 {0000 0001 ... 2222}; moment k starts at kMomOff[k].
+constexpr int kMomOff[5] = {0, 1, 4, 10, 20};
+__host__ __device__ constexpr int sym2(int a, int b) {
+  return a <= b ? a * (5 - a) / 2 + b : b * (5 - b) / 2 + a;
+}
+__host__ __device__ constexpr int sym3(int a, int b, int c) {
+  // sort a <= b <= c
+  return a > b ? sym3(b, a, c) : (b > c ? sym3(a, c, b) : (a == 0 ? 0 : (a == 1 ? 6 : 9)) + sym2(b, c) - sym2(a, a));
+}
+__host__ __device__ constexpr int sym4(int a, int b, int c, int d) {
+  return a > b ? sym4(b, a, c, d)
+               : (b > c ? sym4(a, c, b, d)
+                        : (c > d ? sym4(a, b, d, c)
+                                 : (a == 0 ? 0 : (a == 1 ? 10 : 14)) + sym3(b, c, d) - sym3(a, a, a)));
+}
+// unique index tuples of M2, M3, M4 and their multiplicities in a full contraction
+constexpr int kPair[6][2] = {{0, 0}, {0, 1}, {0, 2}, {1, 1}, {1, 2}, {2, 2}};
+constexpr int kTri[10][3] = {{0, 0, 0}, {0, 0, 1}, {0, 0, 2}, {0, 1, 1}, {0, 1, 2},
+                             {0, 2, 2}, {1, 1, 1}, {1, 1, 2}, {1, 2, 2}, {2, 2, 2}};
+constexpr int kQuad[15][4] = {{0, 0, 0, 0}, {0, 0, 0, 1}, {0, 0, 0, 2}, {0, 0, 1, 1}, {0, 0, 1, 2},
+                              {0, 0, 2, 2}, {0, 1, 1, 1}, {0, 1, 1, 2}, {0, 1, 2, 2}, {0, 2, 2, 2},
+                              {1, 1, 1, 1}, {1, 1, 1, 2}, {1, 1, 2, 2}, {1, 2, 2, 2}, {2, 2, 2, 2}};
+constexpr float kPairMul[6] = {1, 2, 2, 1, 2, 1};
+constexpr float kTriMul[10] = {1, 3, 3, 3, 6, 3, 1, 3, 3, 1};
+
+// Position in m[] of M_k[i, rest] for each free index i and unique tuple `rest`
+// of the remaining k-1 indices, tabulated at compile time so that the unrolled
+// contractions index the moments with constants.
+struct MomIndex {
+  int b2[3][3];
+  int b3[3][6];
+  int b4[3][10];
+};
+constexpr MomIndex make_mom_index() {
+  MomIndex t{};
+  for (int k = 0; k < 3; ++k) {
+    for (int j = 0; j < 3; ++j) t.b2[k][j] = kMomOff[2] + sym2(k, j);
+    for (int p = 0; p < 6; ++p) t.b3[k][p] = kMomOff[3] + sym3(k, kPair[p][0], kPair[p][1]);
+    for (int q = 0; q < 10; ++q) t.b4[k][q] = kMomOff[4] + sym4(k, kTri[q][0], kTri[q][1], kTri[q][2]);
+  }
+  return t;
+}
+constexpr MomIndex kMomIdx = make_mom_index();
+
+// Expansion of G = sum_j w_j y_j / (R^2 - x.y_j) to order P about c from the
+// moments m[35] (T = float: the far kernel, m already divided by 4 pi R;
+// T = double: training targets).  D = (|x-c|^2 + e_c) / 2, e_c = R^2 - |c|^2.
+__device__ __forceinline__ float recip(float v) { return __builtin_amdgcn_rcpf(v); }
+__device__ __forceinline__ double recip(double v) { return 1.0 / v; }
+
+template <int P, typename T>
+__device__ __forceinline__ void expansion(const T c[3], T e_c, const T* m, const T x[3], T G[3]) {
+  const T dx = x[0] - c[0], dy = x[1] - c[1], dz = x[2] - c[2];
+  const T D = T(0.5) * (dx * dx + dy * dy + dz * dz + e_c);
+  const T phi = recip(D);
+  // b_k = M_k[., x^(k-1)], a_k = x . b_k
+  T b[5][3] = {};
+  T a[5] = {};
+  a[0] = m[0];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) b[1][k] = m[1 + k];
+  if (P >= 2) {
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      T s = 0;
+#pragma unroll
+      for (int j = 0; j < 3; ++j) s += m[kMomIdx.b2[k][j]] * x[j];
+      b[2][k] = s;
+    }
+  }
+  if (P >= 3) {
+    T q[6];
+#pragma unroll
+    for (int p = 0; p < 6; ++p) q[p] = T(kPairMul[p]) * x[kPair[p][0]] * x[kPair[p][1]];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      T s = 0;
+#pragma unroll
+      for (int p = 0; p < 6; ++p) s += m[kMomIdx.b3[k][p]] * q[p];
+      b[3][k] = s;
+    }
+  }
+  if (P >= 4) {
+    T q[10];
+#pragma unroll
+    for (int t = 0; t < 10; ++t) q[t] = T(kTriMul[t]) * x[kTri[t][0]] * x[kTri[t][1]] * x[kTri[t][2]];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      T s = 0;
+#pragma unroll
+      for (int t = 0; t < 10; ++t) s += m[kMomIdx.b4[k][t]] * q[t];
+      b[4][k] = s;
+    }
+  }
+#pragma unroll
+  for (int k = 1; k <= P; ++k) a[k] = x[0] * b[k][0] + x[1] * b[k][1] + x[2] * b[k][2];
+  // Horner in phi
+  T al = a[P];
+#pragma unroll
+  for (int k = P - 1; k >= 0; --k) al = a[k] + phi * al;
+  al = al * phi;
+  T be[3];
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    T s = b[P][i];
+#pragma unroll
+    for (int k = P - 1; k >= 1; --k) s = b[k][i] + phi * s;
+    be[i] = s * phi;
+  }
+#pragma unroll
+  for (int i = 0; i < 3; ++i) G[i] = c[i] * al + be[i];
+}
 
 #pragma clang fp contract(off)
 
@@ -60,6 +184,91 @@ __device__ __forceinline__ void project(double& x, double& y, double& z, double 
   x = x * s;
   y = y * s;
   z = z * s;
+}
+
+// ---------------------------------------------------------------- binning
+// Cell of each particle: the generator with the largest dot product (lowest
+// index on ties); sort keys for targets (all particles) and sources (inactive
+// particles get key ncell and sort last).
+__global__ __launch_bounds__(256) void ff_locate(int n, int ncell, const lpm::Vec4<float>* __restrict__ gen,
+                                                 const double* __restrict__ x, const double* __restrict__ y,
+                                                 const double* __restrict__ z, const int* __restrict__ active,
+                                                 int* __restrict__ cellof, int* __restrict__ key_t,
+                                                 int* __restrict__ key_s, int* __restrict__ iota) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float px = (float)x[i], py = (float)y[i], pz = (float)z[i];
+  float best = -3.f;
+  int bs = 0;
+  for (int s = 0; s < ncell; ++s) {
+    const lpm::Vec4<float> g = gen[s];
+    const float d = fmaf(g.a, px, fmaf(g.b, py, g.c * pz));
+    if (d > best) {
+      best = d;
+      bs = s;
+    }
+  }
+  cellof[i] = bs;
+  key_t[i] = bs;
+  key_s[i] = active[i] ? bs : ncell;
+  iota[i] = i;
+}
+
+__device__ __forceinline__ int lower_bound_dev(const int* __restrict__ a, int n, int key) {
+  int lo = 0, hi = n;
+  while (lo < hi) {
+    const int mid = (lo + hi) >> 1;
+    if (a[mid] < key) lo = mid + 1;
+    else hi = mid;
+  }
+  return lo;
+}
+
+// Start of every cell (and the end, index ncell) in the sorted key arrays.
+__global__ __launch_bounds__(256) void ff_bounds(int n, int ncell, const int* __restrict__ tkey,
+                                                 const int* __restrict__ skey, int* __restrict__ tgt_start,
+                                                 int* __restrict__ src_start) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c > ncell) return;
+  tgt_start[c] = lower_bound_dev(tkey, n, c);
+  src_start[c] = lower_bound_dev(skey, n, c);
+}
+
+// Work items (cell, tile) for tiles of tileA and of tileB targets, in cell
+// order; counts[0], counts[1] = their numbers.  One block of 1024 threads.
+__global__ __launch_bounds__(1024) void ff_make_items(int ncell, const int* __restrict__ tgt_start, int tileA,
+                                                      int tileB, int2* __restrict__ items_a,
+                                                      int2* __restrict__ items_b, int* __restrict__ counts) {
+  __shared__ int sa[1024], sb[1024];
+  const int tid = threadIdx.x;
+  const int per = (ncell + 1023) / 1024;
+  const int c0 = min(ncell, tid * per), c1 = min(ncell, c0 + per);
+  int na = 0, nb = 0;
+  for (int c = c0; c < c1; ++c) {
+    const int cnt = tgt_start[c + 1] - tgt_start[c];
+    na += (cnt + tileA - 1) / tileA;
+    nb += (cnt + tileB - 1) / tileB;
+  }
+  sa[tid] = na;
+  sb[tid] = nb;
+  __syncthreads();
+  for (int off = 1; off < 1024; off <<= 1) {  // inclusive scan
+    const int va = tid >= off ? sa[tid - off] : 0, vb = tid >= off ? sb[tid - off] : 0;
+    __syncthreads();
+    sa[tid] += va;
+    sb[tid] += vb;
+    __syncthreads();
+  }
+  int oa = sa[tid] - na, ob = sb[tid] - nb;
+  for (int c = c0; c < c1; ++c) {
+    const int cnt = tgt_start[c + 1] - tgt_start[c];
+    for (int j = 0; j * tileA < cnt; ++j) items_a[oa++] = make_int2(c, j);
+    for (int j = 0; j * tileB < cnt; ++j) items_b[ob++] = make_int2(c, j);
+  }
+  if (tid == 1023) {
+    counts[0] = sa[1023];
+    counts[1] = sb[1023];
+  }
 }
 
 // ---------------------------------------------------------------- packing
@@ -81,12 +290,12 @@ struct GeomArgs {
   const int *src_perm, *src_start, *tgt_perm, *tgt_start;
   const double *x, *y, *z, *q, *area;
   double radius;
-  float* far_rec;      // [ncell][16]: c(3), e_c, M0', M1'(3), M2'(6: xx xy xz yy yz zz), pad
+  float* far_rec;      // [ncell][40]: c(3), e_c, M0..M4 / (4 pi R) (35), rho
   float* mlp_rec;      // [ncell][32]: c(3), 1/l, Q(9), s_out, pad(2), features 3..12, pad(6)
-  const int* anchor;   // [ncell] anchor source (particle index) of each cell's frame
+  const double* adir;  // [ncell][3] unit anchor direction of each cell's frame
   double* geo;         // [ncell][8]: src dir (3), src rho, tgt dir (3), tgt rho
   double* cen64;       // [ncell][3] FP64 source centroids
-  double* mom64;       // [ncell][10]
+  double* mom64;       // [ncell][kNMom]
 };
 
 __device__ double block_sum(double v, double* sh) {
@@ -135,29 +344,31 @@ __global__ __launch_bounds__(256) void ff_cell_geometry(GeomArgs a) {
   const double cx = aa > 0 ? ax / aa : 0.0, cy = aa > 0 ? ay / aa : 0.0, cz = aa > 0 ? az / aa : 1.0;
   const double cn = sqrt(cx * cx + cy * cy + cz * cz);
   const double ux = cx / cn, uy = cy / cn, uz = cz / cn;
-  // moments about c, radius
-  double m[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+  // moments M0..M4 about c, radius
+  double m[kNMom];
+  for (int k = 0; k < kNMom; ++k) m[k] = 0.0;
   double wabs = 0.0, rho = 0.0;
   for (int j = s0 + threadIdx.x; j < s1; j += 256) {
     const int p = a.src_perm[j];
     const double w = a.q[p] * a.area[p];
-    const double dx = a.x[p] - cx, dy = a.y[p] - cy, dz = a.z[p] - cz;
+    const double d[3] = {a.x[p] - cx, a.y[p] - cy, a.z[p] - cz};
     m[0] += w;
-    m[1] += w * dx;
-    m[2] += w * dy;
-    m[3] += w * dz;
-    m[4] += w * dx * dx;
-    m[5] += w * dx * dy;
-    m[6] += w * dx * dz;
-    m[7] += w * dy * dy;
-    m[8] += w * dy * dz;
-    m[9] += w * dz * dz;
+#pragma unroll
+    for (int i = 0; i < 3; ++i) m[1 + i] += w * d[i];
+#pragma unroll
+    for (int i = 0; i < 6; ++i) m[kMomOff[2] + i] += w * d[kPair[i][0]] * d[kPair[i][1]];
+#pragma unroll
+    for (int i = 0; i < 10; ++i) m[kMomOff[3] + i] += w * d[kTri[i][0]] * d[kTri[i][1]] * d[kTri[i][2]];
+#pragma unroll
+    for (int i = 0; i < 15; ++i)
+      m[kMomOff[4] + i] += w * d[kQuad[i][0]] * d[kQuad[i][1]] * d[kQuad[i][2]] * d[kQuad[i][3]];
     wabs += fabs(w);
     const double r = sqrt(a.x[p] * a.x[p] + a.y[p] * a.y[p] + a.z[p] * a.z[p]);
     const double cosang = (a.x[p] * ux + a.y[p] * uy + a.z[p] * uz) / r;
     rho = fmax(rho, acos(fmin(1.0, fmax(-1.0, cosang))));
   }
-  for (int k = 0; k < 10; ++k) m[k] = block_sum(m[k], sh);
+#pragma unroll
+  for (int k = 0; k < kNMom; ++k) m[k] = block_sum(m[k], sh);
   wabs = block_sum(wabs, sh);
   rho = block_max(rho, sh);
   // target cell direction and radius
@@ -195,19 +406,18 @@ __global__ __launch_bounds__(256) void ff_cell_geometry(GeomArgs a) {
     fr[1] = (float)cy;
     fr[2] = (float)cz;
     fr[3] = (float)(R * R - (cx * cx + cy * cy + cz * cz));
-    for (int k = 0; k < 10; ++k) fr[4 + k] = (float)(m[k] * inv);
-    fr[14] = (float)rho;
-    fr[15] = 0.f;
+    for (int k = 0; k < kNMom; ++k) fr[4 + k] = (float)(m[k] * inv);
+    fr[4 + kNMom] = (float)rho;
     const double l = fmax(2.0 * R * sin(0.5 * rho), 1e-12);
     const double W = wabs + 1e-300;
-    // cell frame: e3 = c/|c|, e1 = tangential part of (anchor - c), e2 = e3 x e1
+    // cell frame: e3 = c/|c|, e1 = tangential part of (R anchor - c), e2 = e3 x e1
     double Q[3][3];
     Q[2][0] = ux;
     Q[2][1] = uy;
     Q[2][2] = uz;
     {
-      const int pa = a.anchor[c];
-      double ex = a.x[pa] - cx, ey = a.y[pa] - cy, ez = a.z[pa] - cz;
+      const double* ad = a.adir + (size_t)c * 3;
+      double ex = R * ad[0] - cx, ey = R * ad[1] - cy, ez = R * ad[2] - cz;
       const double dn = ex * ux + ey * uy + ez * uz;
       ex -= dn * ux;
       ey -= dn * uy;
@@ -260,7 +470,7 @@ __global__ __launch_bounds__(256) void ff_cell_geometry(GeomArgs a) {
     a.cen64[(size_t)c * 3 + 0] = cx;
     a.cen64[(size_t)c * 3 + 1] = cy;
     a.cen64[(size_t)c * 3 + 2] = cz;
-    for (int k = 0; k < 10; ++k) a.mom64[(size_t)c * 10 + k] = m[k];
+    for (int k = 0; k < kNMom; ++k) a.mom64[(size_t)c * kNMom + k] = m[k];
   }
 }
 
@@ -329,7 +539,8 @@ struct NearArgs {
   const int* near_list;
   const int* near_count;
   const int2* items;            // (target cell, wave tile)
-  int n_items, ncell;
+  const int* n_items;  // device count
+  int ncell;
   double* out;                  // [3][nt] FP64, cell order
   int nt;
 };
@@ -337,7 +548,7 @@ struct NearArgs {
 __global__ __launch_bounds__(256) void ff_near_kernel(NearArgs a) {
   using P = lpm::SphereBS<float, false>;
   const int item = blockIdx.x * 4 + (threadIdx.x >> 6);
-  if (item >= a.n_items) return;
+  if (item >= *a.n_items) return;
   const int lane = threadIdx.x & 63;
   const int2 it = a.items[item];
   const int t = it.x;
@@ -424,13 +635,15 @@ struct FarArgs {
   const float* cell;          // far records
   const unsigned char* cls;
   const int2* items;
-  int n_items, ncell, nt;
+  const int* n_items;  // device count
+  int ncell, nt;
   double* out;                // [3][nt], accumulated into (+=)
 };
 
+template <int P>
 __global__ __launch_bounds__(256) void ff_far_kernel(FarArgs a) {
   const int item = blockIdx.x * 4 + (threadIdx.x >> 6);
-  if (item >= a.n_items) return;
+  if (item >= *a.n_items) return;
   const int lane = threadIdx.x & 63;
   const int2 it = a.items[item];
   const int t = it.x;
@@ -459,25 +672,16 @@ __global__ __launch_bounds__(256) void ff_far_kernel(FarArgs a) {
     for (int s = s0; s < s1; ++s) {
       if (row[s] == 2) continue;  // near: summed directly (wave-uniform branch)
       const float* cr = a.cell + (size_t)s * kCellRec;
-      float cv[14];
+      constexpr int NV = 4 + kMomOff[P] + (P + 1) * (P + 2) / 2;  // c, e_c, M0..MP
+      float cv[NV];
 #pragma unroll
-      for (int e = 0; e < 14; ++e) cv[e] = cr[e];
+      for (int e = 0; e < NV; ++e) cv[e] = cr[e];
 #pragma unroll
       for (int r = 0; r < kR; ++r) {
-        const float dx = X[r][0] - cv[0], dy = X[r][1] - cv[1], dz = X[r][2] - cv[2];
-        const float D = 0.5f * fmaf(dx, dx, fmaf(dy, dy, fmaf(dz, dz, cv[3])));
-        const float iD = __builtin_amdgcn_rcpf(D);
-        // M2 x
-        const float mx = fmaf(cv[8], X[r][0], fmaf(cv[9], X[r][1], cv[10] * X[r][2]));
-        const float my = fmaf(cv[9], X[r][0], fmaf(cv[11], X[r][1], cv[12] * X[r][2]));
-        const float mz = fmaf(cv[10], X[r][0], fmaf(cv[12], X[r][1], cv[13] * X[r][2]));
-        const float xm1 = fmaf(cv[5], X[r][0], fmaf(cv[6], X[r][1], cv[7] * X[r][2]));
-        const float xm2x = fmaf(mx, X[r][0], fmaf(my, X[r][1], mz * X[r][2]));
-        const float alpha = iD * fmaf(iD, fmaf(iD, xm2x, xm1), cv[4]);
-        const float iD2 = iD * iD;
-        p[r][0] += fmaf(cv[0], alpha, fmaf(cv[5], iD, mx * iD2));
-        p[r][1] += fmaf(cv[1], alpha, fmaf(cv[6], iD, my * iD2));
-        p[r][2] += fmaf(cv[2], alpha, fmaf(cv[7], iD, mz * iD2));
+        float G[3];
+        expansion<P, float>(cv, cv[3], cv + 4, X[r], G);
+#pragma unroll
+        for (int c = 0; c < 3; ++c) p[r][c] += G[c];
       }
     }
 #pragma unroll
@@ -509,10 +713,16 @@ __device__ __forceinline__ float mlp_feature(const float* mr, const float loc[3]
   return 0.f;
 }
 
-// s_out / r^kOutPow for the local position
-__device__ __forceinline__ float mlp_out_scale(const float* mr, const float loc[3]) {
+// r^(P+2) for squared radius r2 and expansion order P in 2..4
+__device__ __forceinline__ float out_pow(float r2, int order) {
+  const float r4 = r2 * r2;
+  return order == 2 ? r4 : (order == 3 ? r4 * sqrtf(r2) : r4 * r2);
+}
+
+// s_out / r^(P+2) for the local position
+__device__ __forceinline__ float mlp_out_scale(const float* mr, const float loc[3], int order) {
   const float r2 = fmaf(loc[0], loc[0], fmaf(loc[1], loc[1], loc[2] * loc[2]));
-  return mr[13] * __builtin_amdgcn_rcpf(r2 * r2);
+  return mr[13] * __builtin_amdgcn_rcpf(out_pow(r2, order));
 }
 
 struct MlpCorrArgs {
@@ -522,7 +732,8 @@ struct MlpCorrArgs {
   const int* band_list;
   const int* band_count;
   const int2* items;  // (target cell, 64-target tile)
-  int n_items, ncell, nt;
+  const int* n_items;  // device count
+  int ncell, nt, order;
   const __bf16* frag;
   const float* bias;
   double* out;        // [3][nt] accumulated into
@@ -538,7 +749,8 @@ __global__ __launch_bounds__(256) void ff_mlp_kernel(MlpCorrArgs a) {
   constexpr int NCT = 4;
   const int lane = threadIdx.x & 63, g = lane >> 4;
   const int waves = gridDim.x * 4;
-  for (int item = blockIdx.x * 4 + (threadIdx.x >> 6); item < a.n_items; item += waves) {
+  const int n_items = *a.n_items;
+  for (int item = blockIdx.x * 4 + (threadIdx.x >> 6); item < n_items; item += waves) {
     const int2 it = a.items[item];
     const int t = it.x;
     const int k0 = a.tgt_start[t] + it.y * 64;
@@ -572,7 +784,7 @@ __global__ __launch_bounds__(256) void ff_mlp_kernel(MlpCorrArgs a) {
       // lanes 0..15 hold the outputs: scale, rotate to world axes (Q^T o), accumulate
 #pragma unroll
       for (int ct = 0; ct < NCT; ++ct) {
-        const float so = mlp_out_scale(mr, loc[ct]);
+        const float so = mlp_out_scale(mr, loc[ct], a.order);
         const float o0 = o[ct][0] * so, o1 = o[ct][1] * so, o2 = o[ct][2] * so;
 #pragma unroll
         for (int c = 0; c < 3; ++c) acc[ct][c] = fmaf(mr[4 + c], o0, fmaf(mr[7 + c], o1, fmaf(mr[10 + c], o2, acc[ct][c])));
@@ -607,13 +819,13 @@ __global__ __launch_bounds__(256) void ff_epilogue(int nt, const int* __restrict
 
 // --------------------------------------------------- training samples
 // Sample q: a random target (particle) and a random band cell of its cell.
-// Inputs: the 13 MLP features; target: Q (G_exact - G_expansion) / (4 pi R) r^4 / s_out,
+// Inputs: the 13 MLP features; target: Q (G_exact - G_expansion) / (4 pi R) r^(P+2) / s_out,
 // the remainder in the cell frame and scaled, in FP64 (G_exact over the cell's
 // sources, with w = zeta A).  ids receives (particle, cell) of each sample.
 struct SampleArgs {
   int n_samples;
   unsigned long long seed;
-  int nt, ncell;
+  int nt, ncell, order;
   const int *tgt_perm, *tgt_cell, *src_perm, *src_start, *band_list, *band_count;
   const double *x, *y, *z, *q, *area;
   const double *cen64, *mom64;
@@ -631,19 +843,12 @@ __device__ __forceinline__ unsigned long long mix64(unsigned long long z) {
   return z ^ (z >> 31);
 }
 
-__device__ void expansion_G(const double* c, const double* m, double R, double x, double y, double z, double G[3]) {
-  const double dx = x - c[0], dy = y - c[1], dz = z - c[2];
-  const double D = 0.5 * (dx * dx + dy * dy + dz * dz + (R * R - (c[0] * c[0] + c[1] * c[1] + c[2] * c[2])));
-  const double iD = 1.0 / D;
-  const double mx = m[4] * x + m[5] * y + m[6] * z;
-  const double my = m[5] * x + m[7] * y + m[8] * z;
-  const double mz = m[6] * x + m[8] * y + m[9] * z;
-  const double xm1 = m[1] * x + m[2] * y + m[3] * z;
-  const double xm2x = mx * x + my * y + mz * z;
-  const double alpha = m[0] * iD + xm1 * iD * iD + xm2x * iD * iD * iD;
-  G[0] = c[0] * alpha + m[1] * iD + mx * iD * iD;
-  G[1] = c[1] * alpha + m[2] * iD + my * iD * iD;
-  G[2] = c[2] * alpha + m[3] * iD + mz * iD * iD;
+__device__ void expansion_fp64(int order, const double* c, const double* m, double R, const double x[3],
+                               double G[3]) {
+  const double e_c = R * R - (c[0] * c[0] + c[1] * c[1] + c[2] * c[2]);
+  if (order == 2) expansion<2, double>(c, e_c, m, x, G);
+  else if (order == 3) expansion<3, double>(c, e_c, m, x, G);
+  else expansion<4, double>(c, e_c, m, x, G);
 }
 
 __global__ __launch_bounds__(256) void ff_make_samples(SampleArgs a) {
@@ -677,7 +882,8 @@ __global__ __launch_bounds__(256) void ff_make_samples(SampleArgs a) {
     }
     double Gm[3];
     const double* c64 = a.cen64 + (size_t)s * 3;
-    expansion_G(c64, a.mom64 + (size_t)s * 10, R, x, y, z, Gm);
+    const double xv[3] = {x, y, z};
+    expansion_fp64(a.order, c64, a.mom64 + (size_t)s * kNMom, R, xv, Gm);
     const float* mr = a.mlp_rec + (size_t)s * kMlpRec;
     // local position in FP64 for the scaling, FP32 (as at inference) for the inputs
     const double il = (double)mr[3];
@@ -687,7 +893,8 @@ __global__ __launch_bounds__(256) void ff_make_samples(SampleArgs a) {
                          (double)mr[6 + 3 * i] * (z - c64[2])) * il;
       r2 += li * li;
     }
-    const double inv = r2 * r2 / (4.0 * kPi * R) / (double)mr[13];
+    const double rp = a.order == 2 ? r2 * r2 : (a.order == 3 ? r2 * r2 * sqrt(r2) : r2 * r2 * r2);
+    const double inv = rp / (4.0 * kPi * R) / (double)mr[13];
     for (int i = 0; i < 3; ++i)
       tgt3[i] = (float)(((double)mr[4 + 3 * i] * (Ge[0] - Gm[0]) + (double)mr[5 + 3 * i] * (Ge[1] - Gm[1]) +
                          (double)mr[6 + 3 * i] * (Ge[2] - Gm[2])) * inv);
@@ -788,11 +995,17 @@ struct lpm_bve_ff {
   lpm_ctx* ctx = nullptr;
   int n = 0, ns = 0, ncell = 0;
   double radius = 1.0, omega = 0.0, r_near = 0.1, r_band = 0.45;
+  int order = 4;  // expansion order of far cells
   DevBuf x, y, z, q, u, v, w, area;
   DevBuf k[4][4];
   DevBuf xi, yi, zi, qi;
   DevBuf pu, pv, pw;  // plain-sum velocity (judge)
-  DevBuf src_perm, src_start, tgt_perm, tgt_start, tgt_cell, anchor;
+  // binning: cell-ordered particle lists rebuilt at every evaluation
+  DevBuf src_perm, src_key, src_start, tgt_perm, tgt_cell, tgt_start;
+  DevBuf key_t, key_s, iota, cellof, active, sort_tmp, item_count;
+  DevBuf gen, adir;  // cell generator directions (float4), frame anchor directions (FP64)
+  int end_bit = 0;
+  size_t sort_bytes = 0;
   DevBuf trec, srec, far_rec, mlp_rec, geo, cen64, mom64;
   DevBuf cls, near_list, near_count, band_list, band_count;
   DevBuf items, mlp_items;
@@ -805,6 +1018,9 @@ struct lpm_bve_ff {
   int64_t n_collected = 0, cap_collected = 0;
   lpm::Mlp net;
   bool trained = false;
+  double ys = 0.0;  // RMS of the training targets (0: not fixed yet)
+  int online_n = 0, online_epochs = 1, online_batch = 4096, online_rounds = 0;
+  double online_lr = 2e-4;
 };
 
 namespace {
@@ -812,6 +1028,30 @@ namespace {
 int ff_upload(lpm_ctx* ctx, DevBuf& b, const void* host, size_t bytes) {
   LPM_HIP(ctx, b.ensure(bytes));
   if (host && bytes) LPM_HIP(ctx, hipMemcpyAsync(b.p, host, bytes, hipMemcpyHostToDevice, ctx->stream));
+  return LPM_OK;
+}
+
+// Bin the particles at (x, y, z) into cells: cell-ordered target and source
+// lists (stable: ascending particle index inside a cell), cell starts, items.
+int ff_bin(lpm_bve_ff* f, const double* x, const double* y, const double* z) {
+  lpm_ctx* ctx = f->ctx;
+  const int n = f->n, nc = f->ncell;
+  hipLaunchKernelGGL(ff_locate, dim3((n + 255) / 256), dim3(256), 0, ctx->stream, n, nc,
+                     f->gen.as<lpm::Vec4<float>>(), x, y, z, f->active.as<int>(), f->cellof.as<int>(),
+                     f->key_t.as<int>(), f->key_s.as<int>(), f->iota.as<int>());
+  size_t bytes = f->sort_bytes;
+  LPM_HIP(ctx, hipcub::DeviceRadixSort::SortPairs(f->sort_tmp.p, bytes, f->key_t.as<int>(), f->tgt_cell.as<int>(),
+                                                  f->iota.as<int>(), f->tgt_perm.as<int>(), n, 0, f->end_bit,
+                                                  ctx->stream));
+  bytes = f->sort_bytes;
+  LPM_HIP(ctx, hipcub::DeviceRadixSort::SortPairs(f->sort_tmp.p, bytes, f->key_s.as<int>(), f->src_key.as<int>(),
+                                                  f->iota.as<int>(), f->src_perm.as<int>(), n, 0, f->end_bit,
+                                                  ctx->stream));
+  hipLaunchKernelGGL(ff_bounds, dim3((nc + 1 + 255) / 256), dim3(256), 0, ctx->stream, n, nc, f->tgt_cell.as<int>(),
+                     f->src_key.as<int>(), f->tgt_start.as<int>(), f->src_start.as<int>());
+  hipLaunchKernelGGL(ff_make_items, dim3(1), dim3(1024), 0, ctx->stream, nc, f->tgt_start.as<int>(), 64 * kR, 64,
+                     f->items.as<int2>(), f->mlp_items.as<int2>(), f->item_count.as<int>());
+  LPM_HIP(ctx, hipGetLastError());
   return LPM_OK;
 }
 
@@ -845,20 +1085,22 @@ int ff_eval(lpm_bve_ff* f, int mode, const double* x, const double* y, const dou
   }
   const int nc = f->ncell;
   dim3 b256(256);
+  if (int rc = ff_bin(f, x, y, z)) return rc;
   hipLaunchKernelGGL(ff_pack_targets, dim3((f->n + 255) / 256), b256, 0, ctx->stream, f->n, f->tgt_perm.as<int>(), x,
                      y, z, f->trec.as<lpm::Vec4<float>>());
   lpm::PackArgs pa{x, y, z, q, nullptr, f->area.as<double>(), nullptr, f->src_perm.as<int>(), 0, 0, f->radius, 1.0};
   lpm::launch_pack(LPM_K_SPHERE_BS, LPM_PREC_FP32_KAHAN, pa, f->srec.p, nullptr, f->ns, 0, ctx->stream);
   GeomArgs ga{nc, f->src_perm.as<int>(), f->src_start.as<int>(), f->tgt_perm.as<int>(), f->tgt_start.as<int>(),
               x, y, z, q, f->area.as<double>(), f->radius, f->far_rec.as<float>(), f->mlp_rec.as<float>(),
-              f->anchor.as<int>(), f->geo.as<double>(), f->cen64.as<double>(), f->mom64.as<double>()};
+              f->adir.as<double>(), f->geo.as<double>(), f->cen64.as<double>(), f->mom64.as<double>()};
   hipLaunchKernelGGL(ff_cell_geometry, dim3(nc), b256, 0, ctx->stream, ga);
-  ClassArgs ca{nc, f->geo.as<double>(), f->r_near, f->r_band, f->cls.as<unsigned char>(), f->near_list.as<int>(),
+  ClassArgs ca{nc, f->geo.as<double>(), f->r_near, f->r_band,
+               f->cls.as<unsigned char>(), f->near_list.as<int>(),
                f->near_count.as<int>(), f->band_list.as<int>(), f->band_count.as<int>()};
   hipLaunchKernelGGL(ff_classify, dim3(nc), b256, 0, ctx->stream, ca);
   NearArgs na{f->trec.as<lpm::Vec4<float>>(), f->tgt_perm.as<int>(), f->tgt_start.as<int>(), f->srec.as<float>(),
               f->src_start.as<int>(), f->near_list.as<int>(), f->near_count.as<int>(), f->items.as<int2>(),
-              f->n_items, nc, f->V.as<double>(), f->n};
+              f->item_count.as<int>(), nc, f->V.as<double>(), f->n};
   hipEvent_t e0 = nullptr, e1 = nullptr;
   if (ctx->timing) {
     e0 = lpm::take_event(ctx);
@@ -867,11 +1109,15 @@ int ff_eval(lpm_bve_ff* f, int mode, const double* x, const double* y, const dou
   }
   hipLaunchKernelGGL(ff_near_kernel, dim3((f->n_items + 3) / 4), b256, 0, ctx->stream, na);
   FarArgs fa{f->trec.as<lpm::Vec4<float>>(), f->tgt_start.as<int>(), f->far_rec.as<float>(), f->cls.as<unsigned char>(),
-             f->items.as<int2>(), f->n_items, nc, f->n, f->V.as<double>()};
-  hipLaunchKernelGGL(ff_far_kernel, dim3((f->n_items + 3) / 4), b256, 0, ctx->stream, fa);
+             f->items.as<int2>(), f->item_count.as<int>(), nc, f->n, f->V.as<double>()};
+  const dim3 gfar((f->n_items + 3) / 4);
+  if (f->order == 2) hipLaunchKernelGGL(ff_far_kernel<2>, gfar, b256, 0, ctx->stream, fa);
+  else if (f->order == 3) hipLaunchKernelGGL(ff_far_kernel<3>, gfar, b256, 0, ctx->stream, fa);
+  else hipLaunchKernelGGL(ff_far_kernel<4>, gfar, b256, 0, ctx->stream, fa);
   if (mode == 2 && f->trained) {
     MlpCorrArgs ma{f->trec.as<lpm::Vec4<float>>(), f->tgt_start.as<int>(), f->mlp_rec.as<float>(),
-                   f->band_list.as<int>(), f->band_count.as<int>(), f->mlp_items.as<int2>(), f->n_mlp_items, nc, f->n,
+                   f->band_list.as<int>(), f->band_count.as<int>(), f->mlp_items.as<int2>(), f->item_count.as<int>() + 1, nc, f->n,
+                   f->order,
                    f->net.frag.as<__bf16>(), f->net.bias.as<float>(), f->V.as<double>()};
     hipLaunchKernelGGL(ff_mlp_kernel, dim3(2 * ctx->num_cus), b256, 0, ctx->stream, ma);
   }
@@ -936,7 +1182,6 @@ extern "C" {
 int lpm_bve_ff_new(lpm_ctx* ctx, lpm_bve_ff** out, int n, const double* x, const double* y, const double* z,
                    const double* relvort, const double* area, const int32_t* active, const int32_t* cell, int ncell,
                    double radius, double rotation_rate, double near_radius, double band_radius, int flags) {
-  (void)flags;
   if (!ctx || !out) return LPM_ERR_ARG;
   ctx->err.clear();
   *out = nullptr;
@@ -944,6 +1189,9 @@ int lpm_bve_ff_new(lpm_ctx* ctx, lpm_bve_ff** out, int n, const double* x, const
     return fail(ctx, LPM_ERR_ARG, "lpm_bve_ff_new: bad arguments");
   if (!(radius > 0.0) || !(near_radius >= 0.0) || !(band_radius >= near_radius))
     return fail(ctx, LPM_ERR_ARG, "lpm_bve_ff_new: need radius > 0 and 0 <= near_radius <= band_radius");
+  const int order = (flags & LPM_FF_ORDER_MASK) == 0 ? 4 : (flags & LPM_FF_ORDER_MASK);
+  if (order < 2 || order > 4 || (flags & ~LPM_FF_ORDER_MASK))
+    return fail(ctx, LPM_ERR_ARG, "lpm_bve_ff_new: flags hold the expansion order 2, 3 or 4 (0: 4) only");
   for (int i = 0; i < n; ++i)
     if (cell[i] < 0 || cell[i] >= ncell) return fail(ctx, LPM_ERR_ARG, "lpm_bve_ff_new: cell id out of range");
   LPM_HIP(ctx, hipSetDevice(ctx->device));
@@ -955,64 +1203,71 @@ int lpm_bve_ff_new(lpm_ctx* ctx, lpm_bve_ff** out, int n, const double* x, const
   f->omega = rotation_rate;
   f->r_near = near_radius;
   f->r_band = band_radius;
-  // cell orderings (stable: ascending particle index inside a cell)
-  std::vector<int> tcount((size_t)ncell + 1, 0), scount((size_t)ncell + 1, 0);
+  f->order = order;
+  // Cells are fixed regions of the sphere: each is the set of points nearest to
+  // its generator, the direction of the area-weighted centroid of the given
+  // cell's active sources at set-up (of all its particles if it has none).  Its
+  // frame anchor is the direction of its active source farthest from that
+  // centroid (lowest particle index on ties).  Particles are binned anew at
+  // every evaluation, so cells never stretch with the flow.
   std::vector<int> plain;
+  std::vector<int32_t> act((size_t)n);
   for (int i = 0; i < n; ++i) {
-    tcount[(size_t)cell[i] + 1]++;
-    if (active[i]) {
-      scount[(size_t)cell[i] + 1]++;
-      plain.push_back(i);
-    }
+    act[(size_t)i] = active[i] ? 1 : 0;
+    if (active[i]) plain.push_back(i);
   }
   f->ns = (int)plain.size();
-  std::partial_sum(tcount.begin(), tcount.end(), tcount.begin());
-  std::partial_sum(scount.begin(), scount.end(), scount.begin());
-  std::vector<int> tperm((size_t)n), sperm((size_t)f->ns), tpos(tcount.begin(), tcount.end() - 1),
-      spos(scount.begin(), scount.end() - 1), tcell((size_t)n);
-  for (int i = 0; i < n; ++i) {
-    const int c = cell[i];
-    tcell[(size_t)tpos[(size_t)c]] = c;
-    tperm[(size_t)tpos[(size_t)c]++] = i;
-    if (active[i]) sperm[(size_t)spos[(size_t)c]++] = i;
-  }
-  // frame anchor of each cell: its active source farthest from the area-weighted
-  // centroid at set-up (lowest particle index on ties), then fixed
-  std::vector<int> anchor((size_t)ncell, -1);
+  std::vector<lpm::Vec4<float>> gen((size_t)ncell);
+  std::vector<double> adir((size_t)ncell * 3, 0.0);
   {
-    std::vector<double> cen((size_t)ncell * 4, 0.0);
-    for (int i = 0; i < n; ++i)
+    std::vector<double> cs((size_t)ncell * 4, 0.0), ca((size_t)ncell * 4, 0.0);
+    for (int i = 0; i < n; ++i) {
+      double* c = &ca[(size_t)cell[i] * 4];
+      c[0] += x[i];
+      c[1] += y[i];
+      c[2] += z[i];
+      c[3] += 1.0;
       if (active[i]) {
-        double* c = &cen[(size_t)cell[i] * 4];
-        c[0] += area[i] * x[i];
-        c[1] += area[i] * y[i];
-        c[2] += area[i] * z[i];
-        c[3] += area[i];
+        double* s = &cs[(size_t)cell[i] * 4];
+        s[0] += area[i] * x[i];
+        s[1] += area[i] * y[i];
+        s[2] += area[i] * z[i];
+        s[3] += area[i];
       }
+    }
     std::vector<double> best((size_t)ncell, -1.0);
     for (int i = 0; i < n; ++i)
       if (active[i]) {
-        const double* c = &cen[(size_t)cell[i] * 4];
+        const double* c = &cs[(size_t)cell[i] * 4];
         const double inv = c[3] > 0 ? 1.0 / c[3] : 0.0;
         const double dx = x[i] - c[0] * inv, dy = y[i] - c[1] * inv, dz = z[i] - c[2] * inv;
         const double d2 = dx * dx + dy * dy + dz * dz;
         if (d2 > best[(size_t)cell[i]]) {
           best[(size_t)cell[i]] = d2;
-          anchor[(size_t)cell[i]] = i;
+          const double r = std::sqrt(x[i] * x[i] + y[i] * y[i] + z[i] * z[i]);
+          adir[(size_t)cell[i] * 3 + 0] = x[i] / r;
+          adir[(size_t)cell[i] * 3 + 1] = y[i] / r;
+          adir[(size_t)cell[i] * 3 + 2] = z[i] / r;
         }
       }
-    for (int c = 0; c < ncell; ++c)
-      if (anchor[(size_t)c] < 0) anchor[(size_t)c] = 0;  // no sources: the frame is never used
+    for (int c = 0; c < ncell; ++c) {
+      const double* s = cs[(size_t)c * 4 + 3] > 0 ? &cs[(size_t)c * 4] : &ca[(size_t)c * 4];
+      const double r = std::sqrt(s[0] * s[0] + s[1] * s[1] + s[2] * s[2]);
+      if (r > 0) gen[(size_t)c] = lpm::Vec4<float>{(float)(s[0] / r), (float)(s[1] / r), (float)(s[2] / r), 0.f};
+      else gen[(size_t)c] = lpm::Vec4<float>{0.f, 0.f, 0.f, 0.f};  // empty cell: never nearest
+    }
   }
-  std::vector<int2> items, mitems;
-  for (int c = 0; c < ncell; ++c) {
-    const int cnt = tcount[(size_t)c + 1] - tcount[(size_t)c];
-    for (int w = 0; w * 64 * kR < cnt; ++w) items.push_back(make_int2(c, w));
-    for (int w = 0; w * 64 < cnt; ++w) mitems.push_back(make_int2(c, w));
-  }
-  f->n_items = (int)items.size();
-  f->n_mlp_items = (int)mitems.size();
+  // work-item upper bounds: per cell ceil(count / tile) <= count / tile + 1
+  f->n_items = ncell + n / (64 * kR) + 1;
+  f->n_mlp_items = ncell + n / 64 + 1;
+  const int end_bit = 32 - __builtin_clz((unsigned)ncell);  // keys 0..ncell
+  size_t sort_bytes = 0;
+  hipcub::DeviceRadixSort::SortPairs(nullptr, sort_bytes, (const int*)nullptr, (int*)nullptr, (const int*)nullptr,
+                                     (int*)nullptr, n, 0, end_bit, ctx->stream);
+  f->end_bit = end_bit;
+  f->sort_bytes = sort_bytes;
   const size_t nb = (size_t)n * sizeof(double);
+  const size_t ni = (size_t)n * sizeof(int);
   int rc = LPM_OK;
   auto chk = [&](int r) {
     if (r && !rc) rc = r;
@@ -1025,22 +1280,27 @@ int lpm_bve_ff_new(lpm_ctx* ctx, lpm_bve_ff** out, int n, const double* x, const
       chk(ff_upload(ctx, f->pv, nullptr, nb)) && chk(ff_upload(ctx, f->pw, nullptr, nb)) &&
       chk(ff_upload(ctx, f->xi, nullptr, nb)) && chk(ff_upload(ctx, f->yi, nullptr, nb)) &&
       chk(ff_upload(ctx, f->zi, nullptr, nb)) && chk(ff_upload(ctx, f->qi, nullptr, nb)) &&
-      chk(ff_upload(ctx, f->src_perm, sperm.data(), std::max<size_t>(sperm.size(), 1) * 4)) &&
-      chk(ff_upload(ctx, f->src_start, scount.data(), scount.size() * 4)) &&
-      chk(ff_upload(ctx, f->tgt_perm, tperm.data(), tperm.size() * 4)) &&
-      chk(ff_upload(ctx, f->tgt_start, tcount.data(), tcount.size() * 4)) &&
-      chk(ff_upload(ctx, f->tgt_cell, tcell.data(), tcell.size() * 4)) &&
-      chk(ff_upload(ctx, f->anchor, anchor.data(), anchor.size() * 4)) &&
+      chk(ff_upload(ctx, f->src_perm, nullptr, ni)) && chk(ff_upload(ctx, f->src_key, nullptr, ni)) &&
+      chk(ff_upload(ctx, f->src_start, nullptr, ((size_t)ncell + 1) * 4)) &&
+      chk(ff_upload(ctx, f->tgt_perm, nullptr, ni)) && chk(ff_upload(ctx, f->tgt_cell, nullptr, ni)) &&
+      chk(ff_upload(ctx, f->tgt_start, nullptr, ((size_t)ncell + 1) * 4)) &&
+      chk(ff_upload(ctx, f->key_t, nullptr, ni)) && chk(ff_upload(ctx, f->key_s, nullptr, ni)) &&
+      chk(ff_upload(ctx, f->iota, nullptr, ni)) && chk(ff_upload(ctx, f->cellof, nullptr, ni)) &&
+      chk(ff_upload(ctx, f->active, act.data(), ni)) &&
+      chk(ff_upload(ctx, f->sort_tmp, nullptr, std::max<size_t>(sort_bytes, 16))) &&
+      chk(ff_upload(ctx, f->gen, gen.data(), (size_t)ncell * sizeof(lpm::Vec4<float>))) &&
+      chk(ff_upload(ctx, f->adir, adir.data(), adir.size() * 8)) &&
+      chk(ff_upload(ctx, f->item_count, nullptr, 2 * 4)) &&
       chk(ff_upload(ctx, f->list_plain, plain.data(), std::max<size_t>(plain.size(), 1) * 4)) &&
-      chk(ff_upload(ctx, f->items, items.data(), std::max<size_t>(items.size(), 1) * sizeof(int2))) &&
-      chk(ff_upload(ctx, f->mlp_items, mitems.data(), std::max<size_t>(mitems.size(), 1) * sizeof(int2))) &&
+      chk(ff_upload(ctx, f->items, nullptr, (size_t)f->n_items * sizeof(int2))) &&
+      chk(ff_upload(ctx, f->mlp_items, nullptr, (size_t)f->n_mlp_items * sizeof(int2))) &&
       chk(ff_upload(ctx, f->trec, nullptr, (size_t)n * 16)) &&
       chk(ff_upload(ctx, f->srec, nullptr, std::max<size_t>((size_t)f->ns, 1) * lpm::SREC * 4)) &&
       chk(ff_upload(ctx, f->far_rec, nullptr, (size_t)ncell * kCellRec * 4)) &&
       chk(ff_upload(ctx, f->mlp_rec, nullptr, (size_t)ncell * kMlpRec * 4)) &&
       chk(ff_upload(ctx, f->geo, nullptr, (size_t)ncell * 8 * 8)) &&
       chk(ff_upload(ctx, f->cen64, nullptr, (size_t)ncell * 3 * 8)) &&
-      chk(ff_upload(ctx, f->mom64, nullptr, (size_t)ncell * 10 * 8)) &&
+      chk(ff_upload(ctx, f->mom64, nullptr, (size_t)ncell * kNMom * 8)) &&
       chk(ff_upload(ctx, f->cls, nullptr, (size_t)ncell * ncell)) &&
       chk(ff_upload(ctx, f->near_list, nullptr, (size_t)ncell * ncell * 4)) &&
       chk(ff_upload(ctx, f->near_count, nullptr, (size_t)ncell * 4)) &&
@@ -1063,6 +1323,8 @@ int lpm_bve_ff_new(lpm_ctx* ctx, lpm_bve_ff** out, int n, const double* x, const
                        f->y.as<double>(), f->z.as<double>());
     chk(ff_eval(f, 0, f->x.as<double>(), f->y.as<double>(), f->z.as<double>(), f->q.as<double>(), f->u.as<double>(),
                 f->v.as<double>(), f->w.as<double>()));
+    // bin once so that the cell queries are defined before any evaluation
+    if (rc == LPM_OK) chk(ff_bin(f, f->x.as<double>(), f->y.as<double>(), f->z.as<double>()));
   }
   if (rc == LPM_OK) {
     hipError_t e = hipStreamSynchronize(ctx->stream);
@@ -1125,7 +1387,7 @@ int lpm_bve_ff_collect(lpm_bve_ff* f, int n_samples, uint64_t seed, int64_t* tot
     nx.p = ny.p = ni.p = nullptr;
     f->cap_collected = cap;
   }
-  SampleArgs sa{n_samples, (unsigned long long)seed, f->n, f->ncell, f->tgt_perm.as<int>(), f->tgt_cell.as<int>(),
+  SampleArgs sa{n_samples, (unsigned long long)seed, f->n, f->ncell, f->order, f->tgt_perm.as<int>(), f->tgt_cell.as<int>(),
                 f->src_perm.as<int>(), f->src_start.as<int>(), f->band_list.as<int>(), f->band_count.as<int>(),
                 f->x.as<double>(), f->y.as<double>(), f->z.as<double>(), f->q.as<double>(), f->area.as<double>(),
                 f->cen64.as<double>(), f->mom64.as<double>(), f->mlp_rec.as<float>(), f->radius,
@@ -1146,10 +1408,31 @@ int lpm_bve_ff_train(lpm_bve_ff* f, int epochs, int batch, double lr, uint64_t s
   if (f->n_collected <= 0) return fail(ctx, LPM_ERR_STATE, "lpm_bve_ff_train: no samples collected");
   if (f->n_collected > INT32_MAX) return fail(ctx, LPM_ERR_ARG, "lpm_bve_ff_train: too many samples");
   LPM_HIP(ctx, hipSetDevice(ctx->device));
+  // targets are trained at unit RMS (scale fixed at the first training); the
+  // stored parameters keep the scale folded into the output layer
+  if (!(f->ys > 0.0)) {
+    double r = 0.0;
+    if (int rc = f->net.rms(f->n_collected * 3, f->sy.as<float>(), &r)) return rc;
+    f->ys = r > 0.0 ? r : 1.0;
+  }
+  if (f->trained)
+    if (int rc = f->net.scale_output((float)(1.0 / f->ys))) return rc;
   if (int rc = f->net.train((int)f->n_collected, f->sx.as<float>(), f->sy.as<float>(), epochs, batch, (float)lr, seed,
-                            final_loss, nullptr, true))
+                            final_loss, nullptr, true, (float)(1.0 / f->ys)))
     return rc;
+  if (int rc = f->net.scale_output((float)f->ys)) return rc;
+  if (int rc = f->net.pack()) return rc;
+  LPM_HIP(ctx, hipStreamSynchronize(ctx->stream));
   f->trained = true;
+  return LPM_OK;
+}
+
+int lpm_bve_ff_set_online(lpm_bve_ff* f, int n_samples, int epochs, int batch, double lr) {
+  if (!f || n_samples < 0 || epochs < 0 || batch <= 0 || !(lr >= 0.0)) return LPM_ERR_ARG;
+  f->online_n = n_samples;
+  f->online_epochs = epochs;
+  f->online_batch = batch;
+  f->online_lr = lr;
   return LPM_OK;
 }
 
@@ -1213,6 +1496,15 @@ int lpm_bve_ff_steps(lpm_bve_ff* f, double dt, int nsteps, int mode, int judge_e
         LPM_HIP(ctx, hipMemcpyAsync(f->u.p, f->pu.p, (size_t)f->n * 8, hipMemcpyDeviceToDevice, ctx->stream));
         LPM_HIP(ctx, hipMemcpyAsync(f->v.p, f->pv.p, (size_t)f->n * 8, hipMemcpyDeviceToDevice, ctx->stream));
         LPM_HIP(ctx, hipMemcpyAsync(f->w.p, f->pw.p, (size_t)f->n * 8, hipMemcpyDeviceToDevice, ctx->stream));
+      } else if (cur == 2 && f->online_n > 0) {
+        // fine-tune on direct sums of the judged state for the steps that follow
+        f->n_collected = 0;
+        if (int rc = lpm_bve_ff_collect(f, f->online_n, 0x5eedull + (uint64_t)(++f->online_rounds), nullptr))
+          return rc;
+        if (f->online_epochs > 0)
+          if (int rc = lpm_bve_ff_train(f, f->online_epochs, f->online_batch, f->online_lr,
+                                        (uint64_t)f->online_rounds, nullptr))
+            return rc;
       }
     }
   }
@@ -1236,6 +1528,33 @@ int lpm_bve_ff_get_state(lpm_bve_ff* f, double* x, double* y, double* z, double*
   return LPM_OK;
 }
 
+int lpm_bve_ff_set_state(lpm_bve_ff* f, const double* x, const double* y, const double* z, const double* relvort) {
+  if (!f || !x || !y || !z || !relvort) return LPM_ERR_ARG;
+  lpm_ctx* ctx = f->ctx;
+  ctx->err.clear();
+  LPM_HIP(ctx, hipSetDevice(ctx->device));
+  const size_t nb = (size_t)f->n * sizeof(double);
+  LPM_HIP(ctx, hipMemcpyAsync(f->x.p, x, nb, hipMemcpyHostToDevice, ctx->stream));
+  LPM_HIP(ctx, hipMemcpyAsync(f->y.p, y, nb, hipMemcpyHostToDevice, ctx->stream));
+  LPM_HIP(ctx, hipMemcpyAsync(f->z.p, z, nb, hipMemcpyHostToDevice, ctx->stream));
+  LPM_HIP(ctx, hipMemcpyAsync(f->q.p, relvort, nb, hipMemcpyHostToDevice, ctx->stream));
+  hipLaunchKernelGGL(ff_project, dim3((f->n + 255) / 256), dim3(256), 0, ctx->stream, f->n, f->radius,
+                     f->x.as<double>(), f->y.as<double>(), f->z.as<double>());
+  if (int rc = ff_eval(f, 0, f->x.as<double>(), f->y.as<double>(), f->z.as<double>(), f->q.as<double>(),
+                       f->u.as<double>(), f->v.as<double>(), f->w.as<double>()))
+    return rc;
+  LPM_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return lpm::collect_timing(ctx);
+}
+
+int lpm_bve_ff_particle_cells(lpm_bve_ff* f, int32_t* cell) {
+  if (!f || !cell) return LPM_ERR_ARG;
+  lpm_ctx* ctx = f->ctx;
+  LPM_HIP(ctx, hipMemcpyAsync(cell, f->cellof.p, (size_t)f->n * 4, hipMemcpyDeviceToHost, ctx->stream));
+  LPM_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return LPM_OK;
+}
+
 int lpm_bve_ff_cells(lpm_bve_ff* f, int* near_cells, int* band_cells) {
   if (!f || !near_cells || !band_cells) return LPM_ERR_ARG;
   lpm_ctx* ctx = f->ctx;
@@ -1253,7 +1572,8 @@ int lpm_bve_ff_delete(lpm_bve_ff* f) {
                    &f->trec, &f->srec, &f->far_rec, &f->mlp_rec, &f->geo, &f->cen64, &f->mom64, &f->cls,
                    &f->near_list, &f->near_count, &f->band_list, &f->band_count, &f->items, &f->mlp_items, &f->V,
                    &f->list_plain, &f->tgt_plain, &f->src_plain, &f->partial_plain, &f->ptrs, &f->err_part, &f->sx,
-                   &f->sy, &f->sid, &f->anchor};
+                   &f->sy, &f->sid, &f->src_key, &f->key_t, &f->key_s, &f->iota, &f->cellof, &f->active,
+                   &f->sort_tmp, &f->item_count, &f->gen, &f->adir};
   for (DevBuf* b : all) b->release();
   for (auto& row : f->k)
     for (auto& b : row) b.release();

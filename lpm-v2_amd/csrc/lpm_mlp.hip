@@ -80,11 +80,32 @@ __global__ __launch_bounds__(256) void adam_step(int n, float lr, float b1, floa
 }
 
 __global__ __launch_bounds__(256) void gather_rows(int n, int cols, const int* __restrict__ idx,
-                                                   const float* __restrict__ src, float* __restrict__ dst) {
+                                                   const float* __restrict__ src, float* __restrict__ dst,
+                                                   float scale) {
   int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= n * cols) return;
   const int r = t / cols, c = t % cols;
-  dst[t] = src[(size_t)idx[r] * cols + c];
+  dst[t] = src[(size_t)idx[r] * cols + c] * scale;
+}
+
+__global__ __launch_bounds__(256) void scale_range(int n, float* __restrict__ p, float s) {
+  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) p[t] *= s;
+}
+
+// sum of squares of n values, one FP64 partial per block (fixed order)
+__global__ __launch_bounds__(256) void sumsq_partial(int64_t n, const float* __restrict__ v, double* __restrict__ part) {
+  __shared__ double sh[256];
+  double s = 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256)
+    s += (double)v[i] * (double)v[i];
+  sh[threadIdx.x] = s;
+  __syncthreads();
+  for (int k = 128; k > 0; k >>= 1) {
+    if ((int)threadIdx.x < k) sh[threadIdx.x] += sh[threadIdx.x + k];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = sh[0];
 }
 
 // Standalone BF16 inference: features [n][nin] FP32 -> out [n][nout].
@@ -305,8 +326,28 @@ int Mlp::adam(float lr) {
   return LPM_OK;
 }
 
+int Mlp::scale_output(float s) {
+  const int n = nout * width + nout;  // last layer: W[nout][width], b[nout]
+  hipLaunchKernelGGL(scale_range, dim3((n + 255) / 256), dim3(256), 0, ctx->stream, n, p.as<float>() + (np - n), s);
+  LPM_HIP(ctx, hipGetLastError());
+  return LPM_OK;
+}
+
+int Mlp::rms(int64_t n, const float* v, double* out) {
+  const int nb = 256;
+  LPM_HIP(ctx, loss_part.ensure(nb * sizeof(double)));
+  hipLaunchKernelGGL(sumsq_partial, dim3(nb), dim3(256), 0, ctx->stream, n, v, loss_part.as<double>());
+  std::vector<double> h(nb);
+  LPM_HIP(ctx, hipMemcpyAsync(h.data(), loss_part.p, nb * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+  LPM_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  double s = 0.0;
+  for (double x : h) s += x;
+  *out = n > 0 ? std::sqrt(s / (double)n) : 0.0;
+  return LPM_OK;
+}
+
 int Mlp::train(int n, const float* X, const float* T, int epochs, int batch, float lr, uint64_t seed,
-               double* final_loss, std::vector<double>* history, bool cosine) {
+               double* final_loss, std::vector<double>* history, bool cosine, float tscale) {
   if (n <= 0 || batch <= 0) return fail(ctx, LPM_ERR_ARG, "mlp train: empty data or batch");
   batch = std::min(batch, n);
   if (int rc = ensure_batch(batch)) return rc;
@@ -325,9 +366,9 @@ int Mlp::train(int n, const float* X, const float* T, int epochs, int batch, flo
     const float lr_ep = cosine ? (float)(0.5 * lr * (1.0 + std::cos(3.14159265358979323846 * ep / epochs))) : lr;
     for (int b0 = 0; b0 + batch <= n; b0 += batch) {
       hipLaunchKernelGGL(gather_rows, dim3((batch * nin + 255) / 256), dim3(256), 0, ctx->stream, batch, nin,
-                         idx.as<int>() + b0, X, xb.as<float>());
+                         idx.as<int>() + b0, X, xb.as<float>(), 1.0f);
       hipLaunchKernelGGL(gather_rows, dim3((batch * nout + 255) / 256), dim3(256), 0, ctx->stream, batch, nout,
-                         idx.as<int>() + b0, T, tb.as<float>());
+                         idx.as<int>() + b0, T, tb.as<float>(), tscale);
       const bool want = (b0 + 2 * batch > n);  // read the loss of the epoch's last batch only
       double l = 0.0;
       if (int rc = loss_grad(batch, xb.as<float>(), tb.as<float>(), want ? &l : nullptr)) return rc;

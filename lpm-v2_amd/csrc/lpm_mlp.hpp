@@ -30,9 +30,12 @@ struct Mlp {
   int forward_f32(int n, const float* X);  // device X [n][nin]; output in act slot nhidden
   int loss_grad(int n, const float* X, const float* T, double* loss);
   int adam(float lr);
-  // cosine: learning rate lr (1 + cos(pi epoch / epochs)) / 2 per epoch, else constant
+  // cosine: learning rate lr (1 + cos(pi epoch / epochs)) / 2 per epoch, else
+  // constant; targets are multiplied by tscale as they are read
   int train(int n, const float* X, const float* T, int epochs, int batch, float lr, uint64_t seed,
-            double* final_loss, std::vector<double>* history, bool cosine = false);
+            double* final_loss, std::vector<double>* history, bool cosine = false, float tscale = 1.0f);
+  int scale_output(float s);                      // last layer W, b *= s
+  int rms(int64_t n, const float* v, double* out);  // sqrt(mean v^2) of device values
   int forward_mfma(int n, const float* X, float* Y);
 };
 

@@ -19,12 +19,15 @@ L.extra_signatures({
     "lpm_bve_ff_velocity": ([_vp, _i, L._pd, L._pd, L._pd], _i),
     "lpm_bve_ff_collect": ([_vp, _i, ctypes.c_uint64, ctypes.POINTER(ctypes.c_int64)], _i),
     "lpm_bve_ff_train": ([_vp, _i, _i, _d, ctypes.c_uint64, L._pd], _i),
+    "lpm_bve_ff_set_online": ([_vp, _i, _i, _i, _d], _i),
     "lpm_bve_ff_samples": ([_vp, ctypes.c_int64, ctypes.c_int64, _pf, _pf, L._pi], _i),
     "lpm_bve_ff_set_params": ([_vp, _pf, ctypes.c_int64], _i),
     "lpm_bve_ff_get_params": ([_vp, _pf, ctypes.c_int64], _i),
     "lpm_bve_ff_steps": ([_vp, _d, _i, _i, _i, _d, ctypes.POINTER(_i), L._pd, ctypes.POINTER(_i)], _i),
     "lpm_bve_ff_get_state": ([_vp, L._pd, L._pd, L._pd, L._pd, L._pd, L._pd, L._pd], _i),
+    "lpm_bve_ff_set_state": ([_vp, L._pd, L._pd, L._pd, L._pd], _i),
     "lpm_bve_ff_cells": ([_vp, L._pi, L._pi], _i),
+    "lpm_bve_ff_particle_cells": ([_vp, L._pi], _i),
     "lpm_bve_ff_delete": ([_vp], _i),
 })
 
@@ -33,10 +36,15 @@ N_PARAMS = 13 * 128 + 128 + 2 * (128 * 128 + 128) + 3 * 128 + 3
 
 
 class FarFieldBVE:
-    def __init__(self, ctx, mesh, relvort, rotation_rate=0.0, near_radius=0.1, band_radius=0.45, radius=1.0):
+    """Sphere BVE whose far cells use an expansion of order `order` (2..4) in
+    their moments, with a network correcting the remainder in the band."""
+
+    def __init__(self, ctx, mesh, relvort, rotation_rate=0.0, near_radius=0.1, band_radius=0.45, radius=1.0,
+                 order=4):
         if mesh.face_cell is None or mesh.cell_level < 0:
             raise ValueError("mesh needs coarse cells (cell_level >= 0)")
         self.ctx = ctx
+        self.order = order
         self.n = mesh.n
         self.ncell = int(mesh.face_cell.max()) + 1
         self.cell = self.particle_cells(mesh)
@@ -44,7 +52,8 @@ class FarFieldBVE:
         act = i32(mesh.active)
         h = ctypes.c_void_p()
         ctx.check(ctx._lib.lpm_bve_ff_new(ctx.handle, ctypes.byref(h), self.n, *[pd(a) for a in arr], pi(act),
-                                          pi(self.cell), self.ncell, radius, rotation_rate, near_radius, band_radius, 0))
+                                          pi(self.cell), self.ncell, radius, rotation_rate, near_radius, band_radius,
+                                          order))
         self.handle = h
         ctx.adopt(self)
 
@@ -83,6 +92,11 @@ class FarFieldBVE:
         self.ctx.check(self.ctx._lib.lpm_bve_ff_train(self.handle, epochs, batch, lr, seed, ctypes.byref(loss)))
         return loss.value
 
+    def set_online(self, n_samples, epochs=1, batch=4096, lr=2e-4):
+        """Fine-tune on n_samples direct-sum samples at every judge that keeps
+        the learned mode (0 turns it off)."""
+        self.ctx.check(self.ctx._lib.lpm_bve_ff_set_online(self.handle, n_samples, epochs, batch, lr))
+
     def params(self):
         p = np.zeros(N_PARAMS, np.float32)
         self.ctx.check(self.ctx._lib.lpm_bve_ff_get_params(self.handle, p.ctypes.data_as(_pf), N_PARAMS))
@@ -104,6 +118,29 @@ class FarFieldBVE:
         out = [np.zeros(self.n) for _ in range(7)]
         self.ctx.check(self.ctx._lib.lpm_bve_ff_get_state(self.handle, *[pd(o) for o in out]))
         return dict(zip(["x", "y", "z", "relvort", "u", "v", "w"], out))
+
+    def set_state(self, x, y, z, relvort):
+        arr = [f64(a) for a in (x, y, z, relvort)]
+        self.ctx.check(self.ctx._lib.lpm_bve_ff_set_state(self.handle, *[pd(a) for a in arr]))
+
+    def pretrain(self, dt, states=5, every=5, samples=2_000_000, epochs=10, batch=4096, lr=1e-3, seed=1):
+        """Train on direct-sum samples of the method's own run: collect at
+        `states` states `every` expansion-mode steps apart, return to the
+        starting state, then train.  Returns the final training loss."""
+        st = self.state()
+        per = max(1, samples // states)
+        for k in range(states):
+            self.collect(per, seed=seed + k)
+            if k + 1 < states:
+                self.steps(dt, every, EXPANSION, judge_every=0)
+        self.set_state(st["x"], st["y"], st["z"], st["relvort"])
+        return self.train(epochs=epochs, batch=batch, lr=lr, seed=seed)
+
+    def binning(self):
+        """Cell of every particle as binned by the last evaluation."""
+        c = np.zeros(self.n, np.int32)
+        self.ctx.check(self.ctx._lib.lpm_bve_ff_particle_cells(self.handle, pi(c)))
+        return c
 
     def cells(self):
         a = np.zeros(self.ncell, np.int32)
