@@ -146,12 +146,31 @@ __global__ __launch_bounds__(256) void mlp_mfma_forward_kernel(int n, int nin, i
   }
 }
 
-void gemm(hipStream_t st, bool ta, bool tb, const GemmArgs& g) {
-  dim3 grid((g.N + 63) / 64, (g.M + 63) / 64), blk(256);
+// Products with a long K and few output tiles (the weight gradients dZ^T H over
+// a minibatch) are split along K when `part` has room (`part_elems` floats).
+void gemm(hipStream_t st, bool ta, bool tb, const GemmArgs& g0, float* part = nullptr, size_t part_elems = 0) {
+  GemmArgs g = g0;
+  g.part = nullptr;
+  g.kchunk = 0;
+  const int tiles = ((g.N + 63) / 64) * ((g.M + 63) / 64);
+  int splits = 1;
+  if (part && tiles < 128 && g.K >= 512 && !g.bias && !g.Z && g.beta == 0.f && g.act == 0) {
+    splits = std::min((256 + tiles - 1) / tiles, g.K / 256);
+    while (splits > 1 && (size_t)splits * g.M * g.N > part_elems) --splits;
+  }
+  if (splits > 1) {
+    g.kchunk = ((g.K + splits - 1) / splits + 15) / 16 * 16;
+    splits = (g.K + g.kchunk - 1) / g.kchunk;
+    g.part = part;
+  }
+  dim3 grid((g.N + 63) / 64, (g.M + 63) / 64, splits), blk(256);
   if (!ta && !tb) hipLaunchKernelGGL((gemm_f32<false, false>), grid, blk, 0, st, g);
   else if (!ta && tb) hipLaunchKernelGGL((gemm_f32<false, true>), grid, blk, 0, st, g);
   else if (ta && !tb) hipLaunchKernelGGL((gemm_f32<true, false>), grid, blk, 0, st, g);
   else hipLaunchKernelGGL((gemm_f32<true, true>), grid, blk, 0, st, g);
+  if (g.part)
+    hipLaunchKernelGGL(gemm_splitk_reduce, dim3((g.M * g.N + 255) / 256), blk, 0, st, splits, g.M, g.N, g.part, g.C,
+                       g.ldc);
 }
 
 }  // namespace
@@ -160,7 +179,7 @@ void gemm(hipStream_t st, bool ta, bool tb, const GemmArgs& g) {
 Mlp::~Mlp() { release(); }
 
 void Mlp::release() {
-  DevBuf* all[] = {&p, &grad, &m, &v, &frag, &bias, &act, &pre, &scratch, &loss_part, &xb, &tb, &idx};
+  DevBuf* all[] = {&p, &grad, &m, &v, &frag, &bias, &act, &pre, &scratch, &loss_part, &splitk, &xb, &tb, &idx};
   for (DevBuf* b : all) b->release();
 }
 
@@ -213,6 +232,8 @@ int Mlp::ensure_batch(int n) {
   LPM_HIP(ctx, pre.ensure((size_t)n * per * 4));
   LPM_HIP(ctx, scratch.ensure((size_t)n * std::max(width, nout) * 2 * 4));
   LPM_HIP(ctx, loss_part.ensure(1024 * 8));
+  // split-K partials of the weight gradients: up to 64 splits of a width x width block
+  LPM_HIP(ctx, splitk.ensure((size_t)64 * std::max(width, nin) * std::max(width, nout) * 4));
   return LPM_OK;
 }
 
@@ -283,7 +304,7 @@ int Mlp::loss_grad(int n, const float* X, const float* T, double* loss) {
     g.ldb = di;
     g.ldc = di;
     g.alpha = 1.f;
-    gemm(ctx->stream, true, false, g);
+    gemm(ctx->stream, true, false, g, splitk.as<float>(), splitk.bytes / sizeof(float));
     hipLaunchKernelGGL(col_sums, dim3(dout), dim3(256), 0, ctx->stream, n, dout, dcur, gb);
     if (l > 0) {
       // dH = dZ W  ([n][dout] x [dout][di]), then times tanh'

@@ -19,7 +19,7 @@ struct Mlp {
   int cur_n = 0;
   DevBuf p, grad, m, v;    // FP32 parameters, gradient, Adam moments
   DevBuf frag, bias;       // BF16 MFMA fragments + FP32 biases (inference)
-  DevBuf act, pre, scratch, loss_part;
+  DevBuf act, pre, scratch, loss_part, splitk;
   DevBuf xb, tb, idx;      // training minibatch staging
   ~Mlp();
   void release();

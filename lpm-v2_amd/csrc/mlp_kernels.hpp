@@ -32,7 +32,10 @@ __device__ __forceinline__ float tanh_fast(float x) {
 // C[M][N] = act(alpha * sum_k A(m, k) B(k, n) + bias[n]) (+ beta C), with
 // A(m, k) = TA ? A[k][m] : A[m][k] and B(k, n) = TB ? B[n][k] : B[k][n].
 // 64 x 64 tile per 256-thread block, 4 x 4 outputs per thread, K tiles of 16
-// staged in LDS.  Z (optional) receives the pre-activation.
+// staged in LDS.  Z (optional) receives the pre-activation.  Split K: with
+// part != nullptr, block z of the grid covers k in [z kchunk, (z+1) kchunk)
+// and writes alpha * its partial product to part[z][M][N] (no bias, beta or
+// activation); gemm_splitk_reduce then sums the partials in a fixed order.
 struct GemmArgs {
   const float* A;
   const float* B;
@@ -43,6 +46,8 @@ struct GemmArgs {
   int lda, ldb, ldc;
   float alpha, beta;
   int act;  // 0 none, 1 tanh
+  float* part;
+  int kchunk;
 };
 
 template <bool TA, bool TB>
@@ -51,17 +56,19 @@ __global__ __launch_bounds__(256) void gemm_f32(GemmArgs g) {
   __shared__ float Bs[16][64 + 1];
   const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
   const int m0 = blockIdx.y * 64, n0 = blockIdx.x * 64;
+  const int kb = g.part ? (int)blockIdx.z * g.kchunk : 0;
+  const int ke = g.part ? min(g.K, kb + g.kchunk) : g.K;
   float acc[4][4] = {};
-  for (int k0 = 0; k0 < g.K; k0 += 16) {
+  for (int k0 = kb; k0 < ke; k0 += 16) {
     for (int t = threadIdx.x; t < 16 * 64; t += 256) {
       const int kk = t / 64, mm = t % 64;
       const int m = m0 + mm, k = k0 + kk;
       float a = 0.f;
-      if (m < g.M && k < g.K) a = TA ? g.A[(size_t)k * g.lda + m] : g.A[(size_t)m * g.lda + k];
+      if (m < g.M && k < ke) a = TA ? g.A[(size_t)k * g.lda + m] : g.A[(size_t)m * g.lda + k];
       As[kk][mm] = a;
       const int n = n0 + mm;
       float b = 0.f;
-      if (n < g.N && k < g.K) b = TB ? g.B[(size_t)n * g.ldb + k] : g.B[(size_t)k * g.ldb + n];
+      if (n < g.N && k < ke) b = TB ? g.B[(size_t)n * g.ldb + k] : g.B[(size_t)k * g.ldb + n];
       Bs[kk][mm] = b;
     }
     __syncthreads();
@@ -79,6 +86,20 @@ __global__ __launch_bounds__(256) void gemm_f32(GemmArgs g) {
     }
     __syncthreads();
   }
+  if (g.part) {
+    float* P = g.part + (size_t)blockIdx.z * g.M * g.N;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int m = m0 + ty * 4 + i;
+      if (m >= g.M) continue;
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const int n = n0 + tx * 4 + j;
+        if (n < g.N) P[(size_t)m * g.N + n] = g.alpha * acc[i][j];
+      }
+    }
+    return;
+  }
 #pragma unroll
   for (int i = 0; i < 4; ++i) {
     const int m = m0 + ty * 4 + i;
@@ -95,6 +116,16 @@ __global__ __launch_bounds__(256) void gemm_f32(GemmArgs g) {
       g.C[(size_t)m * g.ldc + n] = v;
     }
   }
+}
+
+// C[m][n] = sum over splits z = 0, 1, ... of part[z][m][n] (fixed order)
+static __global__ __launch_bounds__(256) void gemm_splitk_reduce(int splits, int M, int N, const float* __restrict__ part,
+                                                                float* __restrict__ C, int ldc) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= M * N) return;
+  float s = 0.f;
+  for (int z = 0; z < splits; ++z) s += part[(size_t)z * M * N + t];
+  C[(size_t)(t / N) * ldc + t % N] = s;
 }
 
 // ------------------------------------------------------- BF16 MFMA forward

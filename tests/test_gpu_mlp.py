@@ -40,12 +40,14 @@ def test_forward_mfma_bf16_matches_oracle(ctx, oracle):
     assert e < 2e-2  # BF16 inputs/weights/activations, FP32 accumulation
 
 
-def test_gradient_matches_oracle(ctx, oracle):
+@pytest.mark.parametrize("n", [257, 4099])
+def test_gradient_matches_oracle(ctx, oracle, n):
     """Hand-written backward (FP32) vs the FP64 oracle's analytic gradient, which
-    is itself checked against central finite differences in the CPU tests."""
+    is itself checked against central finite differences in the CPU tests.
+    n = 4099 takes the split-K path of the weight gradients (ragged last split)."""
     net = Mlp(ctx, NIN, WIDTH, NH, NOUT, seed=5)
     p = net.params()
-    x, t = _data(257, seed=2)
+    x, t = _data(n, seed=2)
     loss, g = net.loss_grad(x, t)
     rl, rg = oracle.mlp_loss_grad(p.astype(np.float64), x.astype(np.float64), t.astype(np.float64), NIN, WIDTH, NH,
                                   NOUT, ACT_TANH)
