@@ -11,8 +11,12 @@ points projected onto the sphere in FP64).
 
 Prints one JSON line.  `value` = G pair-interactions/s over the timed steps.
 Extra fields: RK4 steps/s, velocity error of the FP32 sum against the FP64 GPU
-sum on a block of targets, the pair kernel's roofline, and the CPU baseline
-(the FP64 oracle on the host cores over a bounded block of targets).
+sum on a block of targets, the pair kernel's roofline, the CPU baseline (the
+FP64 oracle on the host cores over a bounded block of targets) and
+`learned_far_field`: the learned far-field form on the same workload (level-3
+cells, trained on 1e7 direct-sum samples of its own run, 100 RK4 steps judged
+by the plain sum every 10 steps), its speedup over the plain steps and the
+largest judged error.
 """
 from __future__ import annotations
 
@@ -41,7 +45,7 @@ def log(*a):
 def build_workload(level: int):
     from lpmgpu import ic, mesh
 
-    m = mesh.icosahedral_mesh(level)
+    m = mesh.icosahedral_mesh(level, cell_level=3)  # level-3 faces: the far-field cells
     x0 = ic.lat_lon_point(0.3, 0.0)
     zeta = ic.gaussian_vortex(m.x, m.y, m.z, x0, 4.0 * np.pi, 16.0) + ic.rh4_vorticity(m.x, m.y, m.z)
     return m, zeta
@@ -77,6 +81,73 @@ def cpu_baseline(m, zeta, target_seconds: float, max_threads: int):
     }
 
 
+def learned_far_field(ctx, m, zeta, omega, dt, plain_ms_per_step, steps, samples, epochs):
+    """The learned far-field form on the same workload (1,280 level-3 cells, near
+    radius 0.15, network band to 0.45, 4th-order cell expansion beyond): train on
+    direct-sum samples of the method's own run, then `steps` RK4 steps with the
+    plain sum judging every 10 steps (fallback above 1e-4) and fine-tuning the
+    network on each judged state.  Returns the speedup over the plain steps."""
+    from lpmgpu import _lib as L
+    from lpmgpu.farfield import EXPANSION, LEARNED, PLAIN, FarFieldBVE
+
+    def rel(a, b):
+        a, b = np.array(a), np.array(b)
+        return float(np.sqrt(((a - b) ** 2).sum(0)).max() / np.sqrt((b**2).sum(0)).max())
+
+    near, band, order = 0.15, 0.45, 4
+    f = FarFieldBVE(ctx, m, zeta, rotation_rate=omega, near_radius=near, band_radius=band, order=order)
+    ctx.reset_kernel_stats()
+    p = f.velocity(PLAIN)
+    _, ms_plain = ctx.kernel_stats(L.K_SPHERE_BS)
+    ctx.reset_kernel_stats()
+    e_exp = rel(f.velocity(EXPANSION), p)
+    _, ms_exp = ctx.kernel_stats(6)
+    t0 = time.perf_counter()
+    loss = f.pretrain(dt, states=5, every=5, samples=samples, epochs=epochs)
+    train_s = time.perf_counter() - t0
+    ctx.reset_kernel_stats()
+    e_net = rel(f.velocity(LEARNED), p)
+    _, ms_net = ctx.kernel_stats(6)
+    nc, bc = f.cells()
+    log(f"[bench] far field: trained in {train_s:.1f}s (loss {loss:.3e}); t=0 error {e_exp:.3e} without the "
+        f"network, {e_net:.3e} with it; one evaluation {ms_plain:.1f} ms plain, {ms_net:.1f} ms learned")
+    f.set_online(250_000, epochs=1, batch=4096, lr=2e-4)
+    judged, mode = [], LEARNED
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    for k in range(steps // 10):
+        r = f.steps(dt, 10, LEARNED, judge_every=10, tol=1e-4)
+        judged.append(r["last_error"])
+        mode = r["mode"]
+        log(f"[bench] far field: steps {10 * (k + 1)} judged error {r['last_error']:.3e} mode {mode}")
+    ctx.synchronize()
+    ms_step = 1e3 * (time.perf_counter() - t0) / max(1, 10 * (steps // 10))
+    f.delete()
+    return {
+        "speedup_vs_plain": round(plain_ms_per_step / ms_step, 3),
+        "max_rel_err": max(judged) if judged else None,
+        "err_definition": "max_i |u_i - u_plain_i| / max_i |u_plain_i| at every judged step (plain = FP32 sum)",
+        "ms_per_step": round(ms_step, 2),
+        "plain_ms_per_step": round(plain_ms_per_step, 2),
+        "steps": 10 * (steps // 10),
+        "judge_every": 10,
+        "tol": 1e-4,
+        "fell_back": mode != LEARNED,
+        "judged_errors": judged,
+        "includes": "judge sums and online fine-tuning (250k samples, 1 epoch) at every judge",
+        "t0_error": {"without_network": e_exp, "with_network": e_net, "gain": round(e_exp / e_net, 3)},
+        "eval_ms": {"plain": round(ms_plain, 2), "expansion": round(ms_exp, 2), "learned": round(ms_net, 2)},
+        "cells": int(f.ncell),
+        "near_radius": near,
+        "band_radius": band,
+        "expansion_order": order,
+        "near_cells_mean": float(nc.mean()),
+        "band_cells_mean": float(bc.mean()),
+        "training": {"samples": samples, "epochs": epochs, "seconds": round(train_s, 1), "loss": loss,
+                     "states": "5 states 5 steps apart of an expansion-mode run, then back to t = 0"},
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -91,6 +162,9 @@ def main():
     ap.add_argument("--workload", default="n655k",
                     help="n655k (default, the headline line) or cfg0/cfg1/cfg2/cfg3 for the other workloads")
     ap.add_argument("--cfg-steps", type=int, default=0, help="override the step count of cfg1/cfg3")
+    ap.add_argument("--ff-steps", type=int, default=100, help="learned far-field RK4 steps (0 = skip)")
+    ap.add_argument("--ff-samples", type=int, default=10_000_000, help="far-field training samples")
+    ap.add_argument("--ff-epochs", type=int, default=10)
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -185,6 +259,13 @@ def main():
     ke, ens, circ = solver.diagnostics()
     solver.delete()
 
+    ff = None
+    if args.ff_steps > 0:
+        log("[bench] learned far field ...")
+        ff = learned_far_field(ctx, m, zeta, omega, args.dt, ms_per_step, args.ff_steps, args.ff_samples,
+                               args.ff_epochs)
+        log(f"[bench] learned far field: {ff['speedup_vs_plain']}x at max error {ff['max_rel_err']}")
+
     cpu = None
     if args.cpu_seconds > 0:
         log("[bench] CPU baseline ...")
@@ -217,6 +298,7 @@ def main():
         "diagnostics": {"ke": ke, "enstrophy": ens, "circulation": circ},
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "learned_far_field": ff,
     }
     print(json.dumps(out), flush=True)
     ctx.close()
