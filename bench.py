@@ -33,9 +33,16 @@ sys.path.insert(0, os.path.join(ROOT, "lpm-v2_amd"))
 sys.path.insert(0, ROOT)
 
 METRIC = "G pair-interactions/s & RK4 steps/s at N=655k; speedup vs plain at stated err"
-FLOPS_PER_PAIR = 23  # sphere Biot-Savart flop contract
+# sphere Biot-Savart flop contract (+, -, x = 1, FMA = 2, rcp = 1, per-source and
+# per-target invariants hoisted): x.y 5, R^2 - x.y 1, 1/den 1, x w_j 1, cross
+# product 9, accumulate 3 x 2 = 23 flops per pair
+FLOPS_PER_PAIR = 23
 PEAK_FP32_TFLOPS = 157.3
 PEAK_HBM_GBS = 8000.0
+# HBM bytes per pair-kernel launch at level 8 from rocprofv3 FETCH_SIZE and
+# WRITE_SIZE runs, scaled by the same counters on a 1 GiB device copy (factors
+# 2.0 and 1.0): profiles/sphere_bs_fp32_icos8_hbm_counters.md
+MEASURED_TRAFFIC_BYTES = {8: 8.88e9}
 
 
 def log(*a):
@@ -165,11 +172,25 @@ def main():
     ap.add_argument("--ff-steps", type=int, default=100, help="learned far-field RK4 steps (0 = skip)")
     ap.add_argument("--ff-samples", type=int, default=10_000_000, help="far-field training samples")
     ap.add_argument("--ff-epochs", type=int, default=10)
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only copy MIB MiB device to device 3 times (calibrates the HBM byte counters) and exit")
     args = ap.parse_args()
 
+    if args.copy_calibration > 0:
+        import torch
+
+        n = args.copy_calibration * (1 << 20) // 4
+        a = torch.ones(n, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_calibration_bytes_each_way": n * 4, "copies": 3}))
+        return
+
     if args.gpus != 1:
-        print(json.dumps({"error": "this build measures one GPU; the direct sum shards by target blocks "
-                          "(see DESIGN.md), --gpus N > 1 is not run"}))
+        print(json.dumps({"error": "this build measures one GPU; the direct sum would shard by target blocks "
+                          "(each GPU all sources, a slice of the targets), --gpus N > 1 is not run"}))
         sys.exit(2)
 
     if args.workload != "n655k":
@@ -250,7 +271,8 @@ def main():
             "peak": PEAK_FP32_TFLOPS,
             "unit": "TFLOP/s",
             "frac": round(achieved / PEAK_FP32_TFLOPS, 4),
-            "traffic": None,
+            "traffic": MEASURED_TRAFFIC_BYTES.get(args.level),
+            "traffic_unit": "bytes per launch (rocprofv3 FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0)",
             "kernel": "pair_sum_kernel<SphereBS<float>>",
             "avg_launch_ms": round(avg_s * 1e3, 3),
             "launches": launches,
