@@ -49,7 +49,8 @@ extern "C" {
 #define LPM_K_SPHERE_GREEN 4 /* sphere stream functions         */
 #define LPM_K_PLANE_GREEN 5  /* planar stream function          */
 #define LPM_K_FAR_FIELD 6    /* learned far field: near + expansion + network */
-#define LPM_K_COUNT 7
+#define LPM_K_PSE_SPHERE8 7  /* sphere PSE Laplacian, 8th-order kernel */
+#define LPM_K_COUNT 8
 
 typedef struct lpm_ctx lpm_ctx;
 
@@ -95,6 +96,16 @@ int lpm_plane_velocity(lpm_ctx *ctx, double *u, double *v, const double *x, cons
 int lpm_pse_plane_laplacian(lpm_ctx *ctx, double *lap, const double *x, const double *y,
                             const double *f, const double *area, const int32_t *active, int n,
                             int i0, int i1, double eps, int order, int precision);
+
+/* PSE Laplacian on the sphere of radius R.  Replaces PSESphereLaplacianAtParticles
+ * (PSEDirectSum.f90:502-532):
+ *   lap_i = eps^-4 sum_{active j} (f_j - f_i) K8(d_ij / eps) A_j
+ * with d_ij the great-circle distance R theta_ij between the directions of x_i
+ * and x_j (SphereDistance, SphereGeometry.f90:107-125), K8 as above. */
+int lpm_pse_sphere_laplacian(lpm_ctx *ctx, double *lap, const double *x, const double *y,
+                             const double *z, const double *f, const double *area,
+                             const int32_t *active, int n, int i0, int i1, double eps,
+                             double radius, int precision);
 
 /* Sphere stream functions.  Replaces SetStreamFunctionsOnMesh (SphereBVE.f90:445-485).
  *   psi_i = sum_{active j != i} -log(R^2 - x_i.x_j) / (4 pi) zeta_j A_j

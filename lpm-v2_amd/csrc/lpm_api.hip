@@ -244,6 +244,43 @@ int lpm_pse_plane_laplacian(lpm_ctx* ctx, double* lap, const double* x, const do
   return download(ctx, i0, i1, {lap});
 }
 
+int lpm_pse_sphere_laplacian(lpm_ctx* ctx, double* lap, const double* x, const double* y, const double* z,
+                             const double* f, const double* area, const int32_t* active, int n, int i0, int i1,
+                             double eps, double radius, int precision) {
+  if (!ctx) return LPM_ERR_ARG;
+  ctx->err.clear();
+  if (int rc = check_block(ctx, n, i0, i1)) return rc;
+  if (int rc = check_prec(ctx, precision)) return rc;
+  if (!lap || !x || !y || !z || !f || !area || !active)
+    return fail(ctx, LPM_ERR_ARG, "lpm_pse_sphere_laplacian: null array argument");
+  if (!(eps > 0.0) || !(radius > 0.0))
+    return fail(ctx, LPM_ERR_ARG, "lpm_pse_sphere_laplacian: eps and radius must be > 0");
+  LPM_HIP(ctx, hipSetDevice(ctx->device));
+  if (i1 == i0) return LPM_OK;
+  int ns = 0;
+  if (int rc = lpm::upload_active_list(ctx, active, n, ctx->list, nullptr, &ns)) return rc;
+  if (int rc = upload(ctx, n, {x, y, z, f, area})) return rc;
+  if (int rc = prepare_outputs(ctx, n, 1)) return rc;
+  lpm::PairProblem pp;
+  pp.kind = LPM_K_PSE_SPHERE8;
+  pp.precision = precision;
+  pp.n = n;
+  pp.i0 = i0;
+  pp.i1 = i1;
+  pp.x = ctx->in[0].as<double>();
+  pp.y = ctx->in[1].as<double>();
+  pp.z = ctx->in[2].as<double>();
+  pp.q0 = ctx->in[3].as<double>();
+  pp.area = ctx->in[4].as<double>();
+  pp.src_list = ctx->list.as<int>();
+  pp.ns = ns;
+  pp.eps = eps;
+  pp.radius = radius;
+  pp.out0 = ctx->out[0].as<double>();
+  if (int rc = lpm::run_pair_sum(ctx, pp)) return rc;
+  return download(ctx, i0, i1, {lap});
+}
+
 int lpm_bve_sphere_stream(lpm_ctx* ctx, double* psi_rel, double* psi_abs, const double* x,
                           const double* y, const double* z, const double* relvort,
                           const double* absvort, const double* area, double radius,

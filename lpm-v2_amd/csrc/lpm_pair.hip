@@ -75,7 +75,20 @@ __global__ __launch_bounds__(256) void pack_sources(PackArgs a, T* __restrict__ 
     else
       r[2] = (T)(a.q0[p] * a.area[p] * (sizeof(T) == 4 ? kLn2 : 1.0) / (4.0 * kPi));
     r[3] = idx_word<T>(p);
-  } else {  // PSE
+  } else if constexpr (KIND == LPM_K_PSE_SPHERE8) {
+    // unit direction times k R (k = a.sigma)
+    const double z = a.z[p];
+    const double kr = a.sigma * a.radius;
+    const double sc = kr / sqrt(x * x + y * y + z * z);
+    r[0] = (T)(x * sc);
+    r[1] = (T)(y * sc);
+    r[2] = (T)(z * sc);
+    r[3] = (T)(a.q0[p] * a.area[p]);
+    r[4] = (T)a.area[p];
+    r[5] = idx_word<T>(p);
+    r[6] = (T)(1.0 / (kr * kr));
+    r[7] = (T)(kr * kr);
+  } else {  // plane PSE
     r[0] = (T)(x * a.sigma);
     r[1] = (T)(y * a.sigma);
     r[2] = (T)(a.q0[p] * a.area[p]);
@@ -104,12 +117,24 @@ __global__ __launch_bounds__(256) void pack_targets(PackArgs a, Vec4<T>* __restr
   } else if constexpr (KIND == LPM_K_PLANE_BS || KIND == LPM_K_PLANE_GREEN) {
     t.a = (T)x;
     t.b = (T)y;
+  } else if constexpr (KIND == LPM_K_PSE_SPHERE8) {
+    const double z = a.z[p];
+    const double sc = a.sigma * a.radius / sqrt(x * x + y * y + z * z);
+    t.a = (T)(x * sc);
+    t.b = (T)(y * sc);
+    t.c = (T)(z * sc);
+    t.d = (T)a.q0[p];
   } else {
     t.a = (T)(x * a.sigma);
     t.b = (T)(y * a.sigma);
     t.c = (T)a.q0[p];
   }
   rec[i] = t;
+}
+
+__global__ __launch_bounds__(256) void zero_fill(size_t n, double* __restrict__ p) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0.0;
 }
 
 // ----------------------------------------------------- green source totals
@@ -176,7 +201,7 @@ __global__ __launch_bounds__(256) void finalize_kernel(FinArgs f) {
   } else if constexpr (KIND == LPM_K_PLANE_BS) {
     f.out0[p] = -A[0];
     f.out1[p] = A[1];
-  } else if constexpr (KIND == LPM_K_PSE8 || KIND == LPM_K_PSE2) {
+  } else if constexpr (KIND == LPM_K_PSE8 || KIND == LPM_K_PSE2 || KIND == LPM_K_PSE_SPHERE8) {
     f.out0[p] = A[0] * f.scale;
   } else if constexpr (KIND == LPM_K_SPHERE_GREEN) {
     if (f.fp32) {
@@ -232,7 +257,10 @@ static KernelFn pick_r(int r) {
   }
 }
 
-static KernelFn pick_kernel(int kind, int prec, int r, bool on_sphere) {
+// variant: sphere kinds, targets on the sphere (no e terms); sphere PSE, the
+// series form of the distance (FP32 only)
+static KernelFn pick_kernel(int kind, int prec, int r, bool variant) {
+  const bool on_sphere = variant;
   if (prec == LPM_PREC_FP32_KAHAN) {
     switch (kind) {
       case LPM_K_SPHERE_BS:
@@ -240,6 +268,8 @@ static KernelFn pick_kernel(int kind, int prec, int r, bool on_sphere) {
       case LPM_K_PLANE_BS: return pick_r<PlaneBS<float>, float>(r);
       case LPM_K_PSE8: return pick_r<PsePolicy<float, 8>, float>(r);
       case LPM_K_PSE2: return pick_r<PsePolicy<float, 2>, float>(r);
+      case LPM_K_PSE_SPHERE8:
+        return variant ? pick_r<PseSphere<float, true>, float>(r) : pick_r<PseSphere<float, false>, float>(r);
       case LPM_K_SPHERE_GREEN:
         return on_sphere ? pick_r<SphereGreen<float, false>, float>(r) : pick_r<SphereGreen<float, true>, float>(r);
       default: return pick_r<PlaneGreen<float>, float>(r);
@@ -251,6 +281,7 @@ static KernelFn pick_kernel(int kind, int prec, int r, bool on_sphere) {
     case LPM_K_PLANE_BS: return pick_r<PlaneBS<double>, double>(r);
     case LPM_K_PSE8: return pick_r<PsePolicy<double, 8>, double>(r);
     case LPM_K_PSE2: return pick_r<PsePolicy<double, 2>, double>(r);
+    case LPM_K_PSE_SPHERE8: return pick_r<PseSphere<double, false>, double>(r);
     case LPM_K_SPHERE_GREEN:
       return on_sphere ? pick_r<SphereGreen<double, false>, double>(r) : pick_r<SphereGreen<double, true>, double>(r);
     default: return pick_r<PlaneGreen<double>, double>(r);
@@ -275,6 +306,7 @@ static void launch_pack_t(int kind, const PackArgs& a, void* srec, void* trec, i
     LPM_PACK_CASE(LPM_K_PLANE_BS)
     LPM_PACK_CASE(LPM_K_PSE8)
     LPM_PACK_CASE(LPM_K_PSE2)
+    LPM_PACK_CASE(LPM_K_PSE_SPHERE8)
     LPM_PACK_CASE(LPM_K_SPHERE_GREEN)
     LPM_PACK_CASE(LPM_K_PLANE_GREEN)
   }
@@ -296,6 +328,7 @@ static void launch_finalize(int kind, const FinArgs& f, hipStream_t st) {
     case LPM_K_PLANE_BS: hipLaunchKernelGGL((finalize_kernel<LPM_K_PLANE_BS, 2>), g, b, 0, st, f); break;
     case LPM_K_PSE8: hipLaunchKernelGGL((finalize_kernel<LPM_K_PSE8, 1>), g, b, 0, st, f); break;
     case LPM_K_PSE2: hipLaunchKernelGGL((finalize_kernel<LPM_K_PSE2, 1>), g, b, 0, st, f); break;
+    case LPM_K_PSE_SPHERE8: hipLaunchKernelGGL((finalize_kernel<LPM_K_PSE_SPHERE8, 1>), g, b, 0, st, f); break;
     case LPM_K_SPHERE_GREEN: hipLaunchKernelGGL((finalize_kernel<LPM_K_SPHERE_GREEN, 2>), g, b, 0, st, f); break;
     default: hipLaunchKernelGGL((finalize_kernel<LPM_K_PLANE_GREEN, 1>), g, b, 0, st, f); break;
   }
@@ -374,7 +407,7 @@ int collect_timing(lpm_ctx* ctx) {
 int run_pair_sum(lpm_ctx* ctx, const PairProblem& pp) {
   const int kind = pp.kind;
   const int prec = pp.precision;
-  if (kind < 0 || kind > LPM_K_PLANE_GREEN) return fail(ctx, LPM_ERR_ARG, "unknown pair kind");
+  if (kind < 0 || kind >= LPM_K_COUNT || kind == LPM_K_FAR_FIELD) return fail(ctx, LPM_ERR_ARG, "unknown pair kind");
   if (prec != LPM_PREC_FP64 && prec != LPM_PREC_FP32_KAHAN)
     return fail(ctx, LPM_ERR_ARG, "precision must be LPM_PREC_FP64 or LPM_PREC_FP32_KAHAN");
   const int nt = pp.i1 - pp.i0;
@@ -395,7 +428,9 @@ int run_pair_sum(lpm_ctx* ctx, const PairProblem& pp) {
 
   const int per_wave = 64 * tu.r;
   const int n_wtiles = (nt + per_wave - 1) / per_wave;
-  KernelFn fn = pick_kernel(kind, prec, tu.r, pp.on_sphere);
+  // sphere PSE in FP32: the series form of the distance when 7 eps <= 0.56 R
+  const bool variant = kind == LPM_K_PSE_SPHERE8 ? (7.0 * pp.eps <= 0.56 * pp.radius) : pp.on_sphere;
+  KernelFn fn = pick_kernel(kind, prec, tu.r, variant);
   int nsplit = 1, chunk = std::max(ns, 1);
   if (ns > 0) {
     int want = tu.splits > 0 ? tu.splits
@@ -410,7 +445,7 @@ int run_pair_sum(lpm_ctx* ctx, const PairProblem& pp) {
   LPM_HIP(ctx, pb.ensure((size_t)nsplit * no * nt * sizeof(double)));
 
   PackArgs pa{pp.x, pp.y, pp.z, pp.q0, pp.q1, pp.area, nullptr, pp.src_list, 0, pp.i0, pp.radius, 1.0};
-  if (kind == LPM_K_PSE8 || kind == LPM_K_PSE2)
+  if (kind == LPM_K_PSE8 || kind == LPM_K_PSE2 || kind == LPM_K_PSE_SPHERE8)
     pa.sigma = (prec == LPM_PREC_FP32_KAHAN ? std::sqrt(kLog2e) : 1.0) / pp.eps;
   launch_pack(kind, prec, pa, sb.p, tb.p, ns, nt, ctx->stream);
   LPM_HIP(ctx, hipGetLastError());
@@ -442,8 +477,11 @@ int run_pair_sum(lpm_ctx* ctx, const PairProblem& pp) {
       ctx->pending.push_back({e0, e1, kind});
     }
   } else {
+    // no sources: zero partials by a kernel (a memset node must not enter a captured graph)
     nsplit = 1;
-    LPM_HIP(ctx, hipMemsetAsync(pb.p, 0, (size_t)no * nt * sizeof(double), ctx->stream));
+    const size_t nz = (size_t)no * nt;
+    hipLaunchKernelGGL(zero_fill, dim3((unsigned)((nz + 255) / 256)), dim3(256), 0, ctx->stream, nz, pb.as<double>());
+    LPM_HIP(ctx, hipGetLastError());
   }
 
   FinArgs f;
@@ -465,9 +503,9 @@ int run_pair_sum(lpm_ctx* ctx, const PairProblem& pp) {
   f.out0 = pp.out0;
   f.out1 = pp.out1;
   f.out2 = pp.out2;
-  if (kind == LPM_K_PSE8 || kind == LPM_K_PSE2) {
+  if (kind == LPM_K_PSE8 || kind == LPM_K_PSE2 || kind == LPM_K_PSE_SPHERE8) {
     const double e4 = pp.eps * pp.eps * pp.eps * pp.eps;
-    f.scale = (kind == LPM_K_PSE8 ? 1.0 : 4.0) / (kPi * e4);
+    f.scale = (kind == LPM_K_PSE2 ? 4.0 : 1.0) / (kPi * e4);
   }
   if (kind == LPM_K_SPHERE_GREEN && f.fp32) {
     LPM_HIP(ctx, ctx->red.ensure(2 * sizeof(double)));

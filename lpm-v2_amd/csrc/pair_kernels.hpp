@@ -148,6 +148,44 @@ struct PsePolicy {
   }
 };
 
+// Sphere PSE Laplacian (PSEDirectSum.f90:502-532): K8 of the great-circle
+// distance R theta between the directions of x_i and x_j (SphereGeometry.f90:
+// atan2(|x_i x x_j|, x_i . x_j) R).  Records hold the unit directions scaled by
+// k R (k = sqrt(log2 e) / eps in FP32, 1 / eps in FP64), so that with
+// u = |x'_i - x'_j|^2 / (k R)^2 = (c / R)^2 (c the chord) the scaled squared
+// distance is q = (k R)^2 theta^2, theta = 2 asin(sqrt(u) / 2).
+// SERIES: theta^2 = u g(u), g = sum_n a_n u^(n-1), a_n = 2 / (n^2 binom(2n, n))
+// (the Taylor series of 4 asin^2(s / 2)), through n = 7; the caller uses it
+// when 7 eps <= 0.56 R, where the first dropped term is below 4e-9 relative.
+// Source record: {x', y', z', f A, A, idx, 1/(kR)^2, (kR)^2}; target {x', y', z', f}.
+template <typename T, bool SERIES>
+struct PseSphere {
+  static constexpr int NOUT = 1;
+  static constexpr int IDX = 5;
+  __device__ __forceinline__ static void arg(const Vec4<T>& t, const T* s, W4<T>& w) {
+    const T dx = t.a - s[0], dy = t.b - s[1], dz = t.c - s[2];
+    const T c2 = fma(dx, dx, fma(dy, dy, dz * dz));
+    const T u = c2 * s[6];
+    if constexpr (SERIES) {
+      T g = fma(u, (T)(1.0 / 84084.0), (T)(1.0 / 16632.0));
+      g = fma(u, g, (T)(1.0 / 3150.0));
+      g = fma(u, g, (T)(1.0 / 560.0));
+      g = fma(u, g, (T)(1.0 / 90.0));
+      g = fma(u, g, (T)(1.0 / 12.0));
+      g = fma(u, g, (T)1.0);
+      w.z = c2 * g;
+    } else {
+      const T th = (T)2.0 * asin(fmin((T)0.5 * sqrt(u), (T)1.0));
+      w.z = th * th * s[7];
+    }
+    w.a = fma(-t.d, s[4], s[3]);  // f_j A_j - f_i A_j
+  }
+  __device__ __forceinline__ static T trans(const W4<T>& w) { return PsePolicy<T, 8>::trans(w); }
+  __device__ __forceinline__ static void acc(const T* s, const W4<T>& w, T e, T a[NOUT]) {
+    PsePolicy<T, 8>::acc(s, w, e, a);
+  }
+};
+
 template <typename T, bool USE_E>
 struct SphereGreen {
   static constexpr int NOUT = 2;

@@ -68,6 +68,17 @@ module lpm_direct_f
       real(c_double), value :: eps
     end function
 
+    integer(c_int) function lpm_pse_sphere_laplacian(ctx, lap, x, y, z, f, area, active, n, i0, i1, eps, &
+        radius, precision) bind(C, name="lpm_pse_sphere_laplacian")
+      import :: c_int, c_ptr, c_double, c_int32_t
+      type(c_ptr), value :: ctx
+      real(c_double) :: lap(*)
+      real(c_double), intent(in) :: x(*), y(*), z(*), f(*), area(*)
+      integer(c_int32_t), intent(in) :: active(*)
+      integer(c_int), value :: n, i0, i1, precision
+      real(c_double), value :: eps, radius
+    end function
+
     integer(c_int) function lpm_bve_sphere_stream(ctx, psi_rel, psi_abs, x, y, z, relvort, absvort, area, &
         radius, active, n, i0, i1, precision) bind(C, name="lpm_bve_sphere_stream")
       import :: c_int, c_ptr, c_double, c_int32_t

@@ -26,7 +26,7 @@ PREC_FP64 = 0
 PREC_FP32_KAHAN = 1
 PRECISIONS = {"fp64": PREC_FP64, "fp32": PREC_FP32_KAHAN, "fp32k": PREC_FP32_KAHAN}
 
-K_SPHERE_BS, K_PLANE_BS, K_PSE8, K_PSE2, K_SPHERE_GREEN, K_PLANE_GREEN = range(6)
+K_SPHERE_BS, K_PLANE_BS, K_PSE8, K_PSE2, K_SPHERE_GREEN, K_PLANE_GREEN, K_FAR_FIELD, K_PSE_SPHERE8 = range(8)
 KIND_NAMES = ["sphere_bs", "plane_bs", "pse8", "pse2", "sphere_green", "plane_green"]
 
 SOLVER_STREAM_EACH_STEP = 1
@@ -57,6 +57,7 @@ _SIGS = {
     "lpm_bve_sphere_velocity": ([_vp, _pd, _pd, _pd, _pd, _pd, _pd, _pd, _pd, _d, _d, _pi, _i, _i, _i, _i], _i),
     "lpm_plane_velocity": ([_vp, _pd, _pd, _pd, _pd, _pd, _pd, _pi, _i, _i, _i, _i], _i),
     "lpm_pse_plane_laplacian": ([_vp, _pd, _pd, _pd, _pd, _pd, _pi, _i, _i, _i, _d, _i, _i], _i),
+    "lpm_pse_sphere_laplacian": ([_vp, _pd, _pd, _pd, _pd, _pd, _pd, _pi, _i, _i, _i, _d, _d, _i], _i),
     "lpm_bve_sphere_stream": ([_vp, _pd, _pd, _pd, _pd, _pd, _pd, _pd, _pd, _d, _pi, _i, _i, _i, _i], _i),
     "lpm_plane_stream": ([_vp, _pd, _pd, _pd, _pd, _pd, _pi, _i, _i, _i, _i], _i),
     "lpm_pse_eps": ([_d, _d, _pd], _i),

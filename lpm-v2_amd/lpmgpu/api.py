@@ -95,6 +95,16 @@ class Context:
                                                      n, i0, i1, eps, order, prec_code(precision)))
         return lap
 
+    def pse_sphere_laplacian(self, x, y, z, f, area, active, eps, radius=1.0, i0=0, i1=None, precision="fp32"):
+        x, y, z, f, a = map(f64, (x, y, z, f, area))
+        act = i32(active)
+        n = x.shape[0]
+        i1 = n if i1 is None else i1
+        lap = np.zeros(n)
+        self.check(self._lib.lpm_pse_sphere_laplacian(self.handle, pd(lap), pd(x), pd(y), pd(z), pd(f), pd(a),
+                                                      pi(act), n, i0, i1, eps, radius, prec_code(precision)))
+        return lap
+
     def bve_sphere_stream(self, x, y, z, relvort, absvort, area, active, radius=1.0, i0=0, i1=None,
                           precision="fp32"):
         x, y, z, qr, qa, a = map(f64, (x, y, z, relvort, absvort, area))
