@@ -90,7 +90,7 @@ def cpu_baseline(m, zeta, target_seconds: float, max_threads: int):
 
 def learned_far_field(ctx, m, zeta, omega, dt, plain_ms_per_step, steps, samples, epochs):
     """The learned far-field form on the same workload (1,280 level-3 cells, near
-    radius 0.15, network band to 0.45, 4th-order cell expansion beyond): train on
+    radius 0.15, network band to 0.30, 4th-order cell expansion beyond): train on
     direct-sum samples of the method's own run, then `steps` RK4 steps with the
     plain sum judging every 10 steps (fallback above 1e-4) and fine-tuning the
     network on each judged state.  Returns the speedup over the plain steps."""
@@ -101,7 +101,7 @@ def learned_far_field(ctx, m, zeta, omega, dt, plain_ms_per_step, steps, samples
         a, b = np.array(a), np.array(b)
         return float(np.sqrt(((a - b) ** 2).sum(0)).max() / np.sqrt((b**2).sum(0)).max())
 
-    near, band, order = 0.15, 0.45, 4
+    near, band, order = 0.15, 0.30, 4
     f = FarFieldBVE(ctx, m, zeta, rotation_rate=omega, near_radius=near, band_radius=band, order=order)
     ctx.reset_kernel_stats()
     p = f.velocity(PLAIN)

@@ -226,15 +226,17 @@ int lpm_bve_batch_delete(lpm_bve_batch *batch);
 
 /* -------------------------------------------- far-field surrogate network -- */
 
-/* tanh MLP nin -> width (x nhidden) -> nout, parameters per layer W[out][in]
- * (row-major) then b[out] in FP32.  Training runs in FP32 on the GPU
+/* MLP nin -> width (x nhidden, activation tanh or ISRU) -> nout, parameters per
+ * layer W[out][in] (row-major) then b[out] in FP32.  Training runs in FP32 on the GPU
  * (hand-written GEMM forward/backward and Adam); inference runs in BF16 on
  * MFMA with FP32 accumulation (nin <= 32, width 128, nhidden 3, nout <= 4).
  * No counterpart in the reference. */
 typedef struct lpm_mlp lpm_mlp;
+#define LPM_ACT_TANH 1 /* hidden activation tanh */
+#define LPM_ACT_ISRU 2 /* hidden activation x / sqrt(1 + x^2) */
 
 int lpm_mlp_new(lpm_ctx *ctx, lpm_mlp **mlp, int nin, int width, int nhidden, int nout,
-                uint64_t seed);
+                int activation, uint64_t seed);
 int lpm_mlp_param_count(lpm_mlp *mlp, int64_t *count);
 int lpm_mlp_set_params(lpm_mlp *mlp, const float *params, int64_t count);
 int lpm_mlp_get_params(lpm_mlp *mlp, float *params, int64_t count);
@@ -263,7 +265,7 @@ int lpm_mlp_delete(lpm_mlp *mlp);
  * points are projected onto the sphere (as with LPM_SOLVER_PROJECT_STAGES).
  * flags: LPM_FF_ORDER(P) with P in 2..4 (0 selects 4).  No counterpart in the
  * reference.
- * The network (13 -> 128 -> 128 -> 128 -> 3, tanh) sees a cell in the cell's
+ * The network (13 -> 128 -> 128 -> 128 -> 3, ISRU) sees a cell in the cell's
  * frame Q (rows e1, e2, e3: e3 along the centroid c, e1 towards the cell's
  * anchor, the active source farthest from c at set-up): inputs Q (x - c) / l,
  * M0 / W, Q M1 / (W l), Q M2 Q^T / (W l^2) (xx xy xz yy yz zz), with moments of

@@ -61,6 +61,8 @@ constexpr int kKT = 256;     // sources per compensated tile
 constexpr int kNMom = 35;    // moments M0..M4: 1 + 3 + 6 + 10 + 15
 constexpr int kCellRec = 40; // floats per far-field cell record
 constexpr int kMlpRec = 32;  // floats per MLP cell record
+constexpr int kMlpNct = 2;   // 16-target column tiles per network pass
+constexpr int kMlpTile = 16 * kMlpNct;  // targets per network work item
 
 // Symmetric moment tensors are stored by their sorted index tuples in
 // lexicographic order: M2 {00 01 02 11 12 22}, M3 {000 001 002 011 012 022 111
@@ -547,7 +549,9 @@ struct NearArgs {
 
 __global__ __launch_bounds__(256) void ff_near_kernel(NearArgs a) {
   using P = lpm::SphereBS<float, false>;
-  const int item = blockIdx.x * 4 + (threadIdx.x >> 6);
+  // the wave's item index is wave-uniform: readfirstlane puts it (and every
+  // index and record address derived from it) in scalar registers
+  const int item = __builtin_amdgcn_readfirstlane(blockIdx.x * 4 + (threadIdx.x >> 6));
   if (item >= *a.n_items) return;
   const int lane = threadIdx.x & 63;
   const int2 it = a.items[item];
@@ -570,7 +574,7 @@ __global__ __launch_bounds__(256) void ff_near_kernel(NearArgs a) {
     for (int c = 0; c < 3; ++c) S[r][c] = C[r][c] = 0.f;
   const int nn = a.near_count[t];
   for (int q = 0; q < nn; ++q) {
-    const int s = a.near_list[(size_t)t * a.ncell + q];
+    const int s = __builtin_amdgcn_readfirstlane(a.near_list[(size_t)t * a.ncell + q]);
     const int j0 = a.src_start[s], j1 = a.src_start[s + 1];
     const bool checked = (s == t);  // a target's own source lives in its own cell
     for (int jt = j0; jt < j1; jt += kKT) {
@@ -642,7 +646,9 @@ struct FarArgs {
 
 template <int P>
 __global__ __launch_bounds__(256) void ff_far_kernel(FarArgs a) {
-  const int item = blockIdx.x * 4 + (threadIdx.x >> 6);
+  // the wave's item index is wave-uniform: readfirstlane puts it (and every
+  // index and record address derived from it) in scalar registers
+  const int item = __builtin_amdgcn_readfirstlane(blockIdx.x * 4 + (threadIdx.x >> 6));
   if (item >= *a.n_items) return;
   const int lane = threadIdx.x & 63;
   const int2 it = a.items[item];
@@ -739,21 +745,24 @@ struct MlpCorrArgs {
   double* out;        // [3][nt] accumulated into
 };
 
-__global__ __launch_bounds__(256) void ff_mlp_kernel(MlpCorrArgs a) {
+// 512 threads: 8 waves (2 per SIMD) share one LDS copy of the weights, which
+// alone takes half a CU's LDS.
+__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(2, 2))) void ff_mlp_kernel(MlpCorrArgs a) {
   __shared__ __attribute__((aligned(16))) __bf16 sfrag[lpm::kMlpFragElems];
   __shared__ float sbias[lpm::kMlpBiasElems];
   for (int t = threadIdx.x; t < lpm::kMlpFragElems / 8; t += blockDim.x)
     reinterpret_cast<lpm::bf16x8_t*>(sfrag)[t] = reinterpret_cast<const lpm::bf16x8_t*>(a.frag)[t];
   for (int t = threadIdx.x; t < lpm::kMlpBiasElems; t += blockDim.x) sbias[t] = a.bias[t];
   __syncthreads();
-  constexpr int NCT = 4;
+  constexpr int NCT = kMlpNct;
   const int lane = threadIdx.x & 63, g = lane >> 4;
-  const int waves = gridDim.x * 4;
+  const int waves = gridDim.x * (blockDim.x >> 6);
   const int n_items = *a.n_items;
-  for (int item = blockIdx.x * 4 + (threadIdx.x >> 6); item < n_items; item += waves) {
+  const int wave0 = __builtin_amdgcn_readfirstlane(blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6));
+  for (int item = wave0; item < n_items; item += waves) {
     const int2 it = a.items[item];
     const int t = it.x;
-    const int k0 = a.tgt_start[t] + it.y * 64;
+    const int k0 = a.tgt_start[t] + it.y * kMlpTile;
     const int k_end = a.tgt_start[t + 1];
     float xs[NCT][3];
 #pragma unroll
@@ -769,18 +778,28 @@ __global__ __launch_bounds__(256) void ff_mlp_kernel(MlpCorrArgs a) {
     for (int ct = 0; ct < NCT; ++ct) acc[ct][0] = acc[ct][1] = acc[ct][2] = 0.f;
     const int nb = a.band_count[t];
     for (int q = 0; q < nb; ++q) {
-      const int s = a.band_list[(size_t)t * a.ncell + q];
+      const int s = __builtin_amdgcn_readfirstlane(a.band_list[(size_t)t * a.ncell + q]);
       const float* mr = a.mlp_rec + (size_t)s * kMlpRec;
+      // the cell's 10 moment features (scalar loads); lane group g holds inputs
+      // 8g .. 8g+7: g = 0 the local position and features 3..7, g = 1 features
+      // 8..12, g = 2, 3 the zero padding -- selected without branches
+      float mv[10];
+#pragma unroll
+      for (int k = 0; k < 10; ++k) mv[k] = mr[16 + k];
       lpm::bf16x8_t in8[NCT];
       float loc[NCT][3];
 #pragma unroll
       for (int ct = 0; ct < NCT; ++ct) {
         mlp_local(mr, xs[ct][0], xs[ct][1], xs[ct][2], loc[ct]);
 #pragma unroll
-        for (int e = 0; e < 8; ++e) in8[ct][e] = (__bf16)mlp_feature(mr, loc[ct], 8 * g + e);
+        for (int e = 0; e < 8; ++e) {
+          const float f0 = e < 3 ? loc[ct][e] : mv[e - 3];
+          const float f1 = e < 5 ? mv[5 + e] : 0.f;
+          in8[ct][e] = (__bf16)(g == 0 ? f0 : (g == 1 ? f1 : 0.f));
+        }
       }
       float o[NCT][4];
-      lpm::mlp_wave_forward<NCT>(in8, sfrag, sbias, o);
+      lpm::mlp_wave_forward<NCT, lpm::kActIsru>(in8, sfrag, sbias, o);
       // lanes 0..15 hold the outputs: scale, rotate to world axes (Q^T o), accumulate
 #pragma unroll
       for (int ct = 0; ct < NCT; ++ct) {
@@ -1049,7 +1068,7 @@ int ff_bin(lpm_bve_ff* f, const double* x, const double* y, const double* z) {
                                                   ctx->stream));
   hipLaunchKernelGGL(ff_bounds, dim3((nc + 1 + 255) / 256), dim3(256), 0, ctx->stream, n, nc, f->tgt_cell.as<int>(),
                      f->src_key.as<int>(), f->tgt_start.as<int>(), f->src_start.as<int>());
-  hipLaunchKernelGGL(ff_make_items, dim3(1), dim3(1024), 0, ctx->stream, nc, f->tgt_start.as<int>(), 64 * kR, 64,
+  hipLaunchKernelGGL(ff_make_items, dim3(1), dim3(1024), 0, ctx->stream, nc, f->tgt_start.as<int>(), 64 * kR, kMlpTile,
                      f->items.as<int2>(), f->mlp_items.as<int2>(), f->item_count.as<int>());
   LPM_HIP(ctx, hipGetLastError());
   return LPM_OK;
@@ -1119,7 +1138,7 @@ int ff_eval(lpm_bve_ff* f, int mode, const double* x, const double* y, const dou
                    f->band_list.as<int>(), f->band_count.as<int>(), f->mlp_items.as<int2>(), f->item_count.as<int>() + 1, nc, f->n,
                    f->order,
                    f->net.frag.as<__bf16>(), f->net.bias.as<float>(), f->V.as<double>()};
-    hipLaunchKernelGGL(ff_mlp_kernel, dim3(2 * ctx->num_cus), b256, 0, ctx->stream, ma);
+    hipLaunchKernelGGL(ff_mlp_kernel, dim3(ctx->num_cus), dim3(512), 0, ctx->stream, ma);
   }
   if (ctx->timing && e0 && e1) {
     LPM_HIP(ctx, hipEventRecord(e1, ctx->stream));
@@ -1259,7 +1278,7 @@ int lpm_bve_ff_new(lpm_ctx* ctx, lpm_bve_ff** out, int n, const double* x, const
   }
   // work-item upper bounds: per cell ceil(count / tile) <= count / tile + 1
   f->n_items = ncell + n / (64 * kR) + 1;
-  f->n_mlp_items = ncell + n / 64 + 1;
+  f->n_mlp_items = ncell + n / kMlpTile + 1;
   const int end_bit = 32 - __builtin_clz((unsigned)ncell);  // keys 0..ncell
   size_t sort_bytes = 0;
   hipcub::DeviceRadixSort::SortPairs(nullptr, sort_bytes, (const int*)nullptr, (int*)nullptr, (const int*)nullptr,
@@ -1316,7 +1335,7 @@ int lpm_bve_ff_new(lpm_ctx* ctx, lpm_bve_ff** out, int n, const double* x, const
     hipError_t e = hipStreamSynchronize(ctx->stream);
     if (e != hipSuccess) rc = lpm::hip_fail(ctx, e, "hipStreamSynchronize");
   }
-  if (rc == LPM_OK) chk(f->net.init(ctx, 13, 128, 3, 3, 12345));
+  if (rc == LPM_OK) chk(f->net.init(ctx, 13, 128, 3, 3, LPM_ACT_ISRU, 12345));
   if (rc == LPM_OK) {
     // start on the sphere, with the plain velocity stored (as BVESolver New)
     hipLaunchKernelGGL(ff_project, dim3((n + 255) / 256), dim3(256), 0, ctx->stream, n, radius, f->x.as<double>(),

@@ -26,9 +26,14 @@ int64_t mlp_param_count(int nin, int width, int nhidden, int nout) {
 
 namespace {
 
-__global__ __launch_bounds__(256) void tanh_backward(int n, const float* __restrict__ H, float* __restrict__ dH) {
+// dZ = dH act'(Z) from the output H alone: tanh' = 1 - H^2;
+// ISRU' = (1 + Z^2)^(-3/2) = (1 - H^2)^(3/2)
+__global__ __launch_bounds__(256) void act_backward(int n, int act, const float* __restrict__ H,
+                                                    float* __restrict__ dH) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) dH[i] = dH[i] * (1.0f - H[i] * H[i]);
+  if (i >= n) return;
+  const float t = 1.0f - H[i] * H[i];
+  dH[i] = dH[i] * (act == kActIsru ? t * sqrtf(fmaxf(t, 0.0f)) : t);
 }
 
 // dY = 2 (Y - T) / (B nout); per-block partial loss (fixed order -> deterministic)
@@ -109,6 +114,7 @@ __global__ __launch_bounds__(256) void sumsq_partial(int64_t n, const float* __r
 }
 
 // Standalone BF16 inference: features [n][nin] FP32 -> out [n][nout].
+template <int ACT>
 __global__ __launch_bounds__(256) void mlp_mfma_forward_kernel(int n, int nin, int nout, const float* __restrict__ X,
                                                                const __bf16* __restrict__ frag,
                                                                const float* __restrict__ bias, float* __restrict__ Y) {
@@ -134,7 +140,7 @@ __global__ __launch_bounds__(256) void mlp_mfma_forward_kernel(int n, int nin, i
       }
     }
     float out[NCT][4];
-    mlp_wave_forward<NCT>(in8, sfrag, sbias, out);
+    mlp_wave_forward<NCT, ACT>(in8, sfrag, sbias, out);
     if (g == 0) {
 #pragma unroll
       for (int ct = 0; ct < NCT; ++ct) {
@@ -183,8 +189,9 @@ void Mlp::release() {
   for (DevBuf* b : all) b->release();
 }
 
-int Mlp::init(lpm_ctx* c, int nin_, int width_, int nhidden_, int nout_, uint64_t seed) {
+int Mlp::init(lpm_ctx* c, int nin_, int width_, int nhidden_, int nout_, int act_, uint64_t seed) {
   ctx = c;
+  act_kind = act_;
   nin = nin_;
   width = width_;
   nhidden = nhidden_;
@@ -261,7 +268,7 @@ int Mlp::forward_f32(int n, const float* X) {
     g.ldc = dout;
     g.alpha = 1.f;
     g.beta = 0.f;
-    g.act = l == nhidden ? 0 : 1;
+    g.act = l == nhidden ? 0 : act_kind;
     gemm(ctx->stream, false, true, g);
     in = g.C;
     din = dout;
@@ -320,7 +327,7 @@ int Mlp::loss_grad(int n, const float* X, const float* T, double* loss) {
       h.ldc = di;
       h.alpha = 1.f;
       gemm(ctx->stream, false, false, h);
-      hipLaunchKernelGGL(tanh_backward, dim3((n * di + 255) / 256), dim3(256), 0, ctx->stream, n * di,
+      hipLaunchKernelGGL(act_backward, dim3((n * di + 255) / 256), dim3(256), 0, ctx->stream, n * di, act_kind,
                          act.as<float>() + (size_t)(l - 1) * slot, dnext);
       std::swap(dcur, dnext);
     }
@@ -411,8 +418,12 @@ int Mlp::train(int n, const float* X, const float* T, int epochs, int batch, flo
 int Mlp::forward_mfma(int n, const float* X, float* Y) {
   if (!mfma_ok()) return fail(ctx, LPM_ERR_UNSUPPORTED, "MFMA forward needs nin <= 32, width 128, 3 hidden layers, nout <= 4");
   const int blocks = std::max(1, std::min(2 * ctx->num_cus, (n + 255) / 256));
-  hipLaunchKernelGGL(mlp_mfma_forward_kernel, dim3(blocks), dim3(256), 0, ctx->stream, n, nin, nout, X,
-                     frag.as<__bf16>(), bias.as<float>(), Y);
+  if (act_kind == kActIsru)
+    hipLaunchKernelGGL(mlp_mfma_forward_kernel<kActIsru>, dim3(blocks), dim3(256), 0, ctx->stream, n, nin, nout, X,
+                       frag.as<__bf16>(), bias.as<float>(), Y);
+  else
+    hipLaunchKernelGGL(mlp_mfma_forward_kernel<kActTanh>, dim3(blocks), dim3(256), 0, ctx->stream, n, nin, nout, X,
+                       frag.as<__bf16>(), bias.as<float>(), Y);
   LPM_HIP(ctx, hipGetLastError());
   return LPM_OK;
 }
@@ -427,15 +438,18 @@ struct lpm_mlp {
 
 extern "C" {
 
-int lpm_mlp_new(lpm_ctx* ctx, lpm_mlp** out, int nin, int width, int nhidden, int nout, uint64_t seed) {
+int lpm_mlp_new(lpm_ctx* ctx, lpm_mlp** out, int nin, int width, int nhidden, int nout, int activation,
+                uint64_t seed) {
   if (!ctx || !out) return LPM_ERR_ARG;
   ctx->err.clear();
   *out = nullptr;
   if (nin <= 0 || width <= 0 || nhidden <= 0 || nout <= 0 || nhidden > 16)
     return fail(ctx, LPM_ERR_ARG, "lpm_mlp_new: bad dimensions");
+  if (activation != LPM_ACT_TANH && activation != LPM_ACT_ISRU)
+    return fail(ctx, LPM_ERR_ARG, "lpm_mlp_new: activation must be LPM_ACT_TANH or LPM_ACT_ISRU");
   LPM_HIP(ctx, hipSetDevice(ctx->device));
   lpm_mlp* m = new lpm_mlp();
-  if (int rc = m->net.init(ctx, nin, width, nhidden, nout, seed)) {
+  if (int rc = m->net.init(ctx, nin, width, nhidden, nout, activation, seed)) {
     delete m;
     return rc;
   }

@@ -14,6 +14,7 @@ int64_t mlp_param_count(int nin, int width, int nhidden, int nout);
 struct Mlp {
   lpm_ctx* ctx = nullptr;
   int nin = 0, width = 0, nhidden = 0, nout = 0;
+  int act_kind = 1;  // hidden-layer activation: 1 tanh, 2 ISRU
   int64_t np = 0;
   int adam_t = 0;
   int cur_n = 0;
@@ -23,7 +24,7 @@ struct Mlp {
   DevBuf xb, tb, idx;      // training minibatch staging
   ~Mlp();
   void release();
-  int init(lpm_ctx* c, int nin, int width, int nhidden, int nout, uint64_t seed);
+  int init(lpm_ctx* c, int nin, int width, int nhidden, int nout, int act, uint64_t seed);
   bool mfma_ok() const;
   int pack();  // FP32 params -> BF16 fragments
   int ensure_batch(int n);

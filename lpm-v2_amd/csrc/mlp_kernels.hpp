@@ -1,15 +1,16 @@
 // mlp_kernels.hpp -- the far-field surrogate network on gfx950.
 //
-// Network: nin -> width (tanh) -> ... (nhidden tanh layers) -> nout, parameters
-// per layer W[out][in] (row-major) then b[out], FP32 master copy.
+// Network: nin -> width (act) -> ... (nhidden act layers) -> nout, act = tanh or
+// ISRU x / sqrt(1 + x^2), parameters per layer W[out][in] (row-major) then
+// b[out], FP32 master copy.
 //
 // Training path (FP32, VALU): tiled GEMMs C = A op(B) with a fused
-// bias + tanh epilogue, the backward GEMMs, column sums and Adam.
+// bias + activation epilogue, the backward GEMMs, column sums and Adam.
 // Inference path (BF16 MFMA, FP32 accumulate), specialised for the
 // 13 -> 128 -> 128 -> 128 -> 3 surrogate: activations are kept transposed
 // (features = MFMA rows, samples = MFMA columns), so the D registers of one
 // layer (lane l: rows 4(l/16)+v of each 16-row tile, column l%16) are, after
-// bias + tanh + BF16 conversion, exactly the B operand of the next layer once
+// bias + activation + BF16 conversion, exactly the B operand of the next layer once
 // the next layer's weight fragments are stored with the matching K order:
 // k-slot e of lane group g in k-step ks is feature 32 ks + 16 (e/4) + 4 g + e%4.
 #pragma once
@@ -26,6 +27,19 @@ __device__ __forceinline__ float tanh_fast(float x) {
   // tanh x = 1 - 2 / (exp(2x) + 1), exp via v_exp_f32 (2^y); saturates cleanly
   const float e = __builtin_amdgcn_exp2f(2.8853900817779268f * x);  // 2 log2(e) x
   return 1.0f - 2.0f * __builtin_amdgcn_rcpf(e + 1.0f);
+}
+
+// Hidden-layer activations: 1 tanh, 2 ISRU x / sqrt(1 + x^2) (one transcendental,
+// v_rsq_f32, instead of tanh's two).
+constexpr int kActTanh = 1;
+constexpr int kActIsru = 2;
+
+template <int ACT>
+__device__ __forceinline__ float act_fast(float x) {
+  if constexpr (ACT == kActIsru)
+    return x * __builtin_amdgcn_rsqf(fmaf(x, x, 1.0f));
+  else
+    return tanh_fast(x);
 }
 
 // ------------------------------------------------------------ FP32 GEMM
@@ -45,7 +59,7 @@ struct GemmArgs {
   int M, N, K;
   int lda, ldb, ldc;
   float alpha, beta;
-  int act;  // 0 none, 1 tanh
+  int act;  // 0 none, kActTanh, kActIsru
   float* part;
   int kchunk;
 };
@@ -112,7 +126,8 @@ __global__ __launch_bounds__(256) void gemm_f32(GemmArgs g) {
       if (g.bias) v += g.bias[n];
       if (g.beta != 0.f) v += g.beta * g.C[(size_t)m * g.ldc + n];
       if (g.Z) g.Z[(size_t)m * g.ldc + n] = v;
-      if (g.act == 1) v = tanhf(v);
+      if (g.act == kActTanh) v = tanhf(v);
+      else if (g.act == kActIsru) v = v / sqrtf(fmaf(v, v, 1.0f));
       g.C[(size_t)m * g.ldc + n] = v;
     }
   }
@@ -190,7 +205,7 @@ static __global__ __launch_bounds__(256) void mlp_pack_fragments(const float* __
 // features (features 8 g .. 8 g + 7 of sample 16 ct + l % 16, g = l / 16);
 // on return out[ct][0..2] are the outputs of that sample in lanes 0..15
 // (lanes 16..63 hold padding rows).  Weights/biases in LDS (fragment order).
-template <int NCT>
+template <int NCT, int ACT = kActTanh>
 __device__ __forceinline__ void mlp_wave_forward(const bf16x8_t in8[NCT], const __bf16* __restrict__ sfrag,
                                                  const float* __restrict__ sbias, float out[NCT][4]) {
   const int lane = threadIdx.x & 63;
@@ -223,7 +238,7 @@ __device__ __forceinline__ void mlp_wave_forward(const bf16x8_t in8[NCT], const 
         for (int e = 0; e < 8; ++e) {
           const int rt = 2 * ks + e / 4, vv = e % 4;
           const float z = acc[rt][ct][vv] + b[16 * rt + 4 * g + vv];
-          v[e] = (__bf16)tanh_fast(z);
+          v[e] = (__bf16)act_fast<ACT>(z);
         }
         h[ks][ct] = v;
       }

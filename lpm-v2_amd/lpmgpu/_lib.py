@@ -92,7 +92,7 @@ _SIGS = {
     "lpm_bve_batch_autotune": ([_vp, _pi, _pi, _pi, _pd], _i),
     "lpm_bve_batch_pairs": ([_vp, ctypes.POINTER(ctypes.c_int64)], _i),
     "lpm_bve_batch_delete": ([_vp], _i),
-    "lpm_mlp_new": ([_vp, ctypes.POINTER(_vp), _i, _i, _i, _i, ctypes.c_uint64], _i),
+    "lpm_mlp_new": ([_vp, ctypes.POINTER(_vp), _i, _i, _i, _i, _i, ctypes.c_uint64], _i),
     "lpm_mlp_param_count": ([_vp, ctypes.POINTER(ctypes.c_int64)], _i),
     "lpm_mlp_set_params": ([_vp, ctypes.POINTER(ctypes.c_float), ctypes.c_int64], _i),
     "lpm_mlp_get_params": ([_vp, ctypes.POINTER(ctypes.c_float), ctypes.c_int64], _i),

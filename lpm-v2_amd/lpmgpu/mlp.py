@@ -16,12 +16,15 @@ def _f32(a):
     return np.ascontiguousarray(a, dtype=np.float32)
 
 
+ACT_TANH, ACT_ISRU = 1, 2
+
+
 class Mlp:
-    def __init__(self, ctx, nin=13, width=128, nhidden=3, nout=3, seed=1):
+    def __init__(self, ctx, nin=13, width=128, nhidden=3, nout=3, seed=1, act=ACT_TANH):
         self.ctx = ctx
-        self.nin, self.width, self.nhidden, self.nout = nin, width, nhidden, nout
+        self.nin, self.width, self.nhidden, self.nout, self.act = nin, width, nhidden, nout, act
         h = ctypes.c_void_p()
-        ctx.check(ctx._lib.lpm_mlp_new(ctx.handle, ctypes.byref(h), nin, width, nhidden, nout, seed))
+        ctx.check(ctx._lib.lpm_mlp_new(ctx.handle, ctypes.byref(h), nin, width, nhidden, nout, act, seed))
         self.handle = h
         ctx.adopt(self)
         n = ctypes.c_int64()

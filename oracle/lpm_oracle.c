@@ -431,9 +431,16 @@ long lpm_oracle_mlp_param_count(int nin, int width, int nhidden, int nout) {
   return c;
 }
 
-static double act_f(int act, double a) { return act == 1 ? tanh(a) : (a > 0.0 ? a : 0.0); }
+/* act: 0 ReLU, 1 tanh, 2 ISRU a / sqrt(1 + a^2) (derivative (1 + a^2)^(-3/2)) */
+static double act_f(int act, double a) {
+  if (act == 1) return tanh(a);
+  if (act == 2) return a / sqrt(1.0 + a * a);
+  return a > 0.0 ? a : 0.0;
+}
 static double act_df(int act, double a, double h) {
-  return act == 1 ? 1.0 - h * h : (a > 0.0 ? 1.0 : 0.0);
+  if (act == 1) return 1.0 - h * h;
+  if (act == 2) return 1.0 / ((1.0 + a * a) * sqrt(1.0 + a * a));
+  return a > 0.0 ? 1.0 : 0.0;
 }
 
 /* Layer l maps dims[l] -> dims[l+1]; activation after every layer but the last. */

@@ -4,11 +4,12 @@ and training by the GPU's own Adam."""
 import numpy as np
 import pytest
 
-from lpmgpu.mlp import Mlp
+from lpmgpu.mlp import ACT_ISRU, ACT_TANH, Mlp
 
 pytestmark = pytest.mark.gpu
 
-NIN, WIDTH, NH, NOUT, ACT_TANH = 13, 128, 3, 3, 1
+NIN, WIDTH, NH, NOUT = 13, 128, 3, 3
+ACTS = [ACT_TANH, ACT_ISRU]  # oracle activation ids 1, 2
 
 
 def _data(n, seed=0):
@@ -18,39 +19,42 @@ def _data(n, seed=0):
     return x, t.astype(np.float32)
 
 
-def test_forward_fp32_matches_oracle(ctx, oracle):
-    net = Mlp(ctx, NIN, WIDTH, NH, NOUT, seed=3)
+@pytest.mark.parametrize("act", ACTS)
+def test_forward_fp32_matches_oracle(ctx, oracle, act):
+    net = Mlp(ctx, NIN, WIDTH, NH, NOUT, seed=3, act=act)
     p = net.params()
     x, _ = _data(300)
-    ref = oracle.mlp_forward(p.astype(np.float64), x.astype(np.float64), NIN, WIDTH, NH, NOUT, ACT_TANH)
+    ref = oracle.mlp_forward(p.astype(np.float64), x.astype(np.float64), NIN, WIDTH, NH, NOUT, act)
     got = net.forward(x)
     e = np.abs(got - ref).max() / np.abs(ref).max()
     print("fp32 forward rel err", e)
     assert e < 1e-5
 
 
-def test_forward_mfma_bf16_matches_oracle(ctx, oracle):
-    net = Mlp(ctx, NIN, WIDTH, NH, NOUT, seed=4)
+@pytest.mark.parametrize("act", ACTS)
+def test_forward_mfma_bf16_matches_oracle(ctx, oracle, act):
+    net = Mlp(ctx, NIN, WIDTH, NH, NOUT, seed=4, act=act)
     p = net.params()
     x, _ = _data(1000, seed=1)
-    ref = oracle.mlp_forward(p.astype(np.float64), x.astype(np.float64), NIN, WIDTH, NH, NOUT, ACT_TANH)
+    ref = oracle.mlp_forward(p.astype(np.float64), x.astype(np.float64), NIN, WIDTH, NH, NOUT, act)
     got = net.forward(x, mfma=True)
     e = np.sqrt(((got - ref) ** 2).sum() / (ref**2).sum())
     print("bf16 mfma forward rel rms err", e)
     assert e < 2e-2  # BF16 inputs/weights/activations, FP32 accumulation
 
 
+@pytest.mark.parametrize("act", ACTS)
 @pytest.mark.parametrize("n", [257, 4099])
-def test_gradient_matches_oracle(ctx, oracle, n):
+def test_gradient_matches_oracle(ctx, oracle, n, act):
     """Hand-written backward (FP32) vs the FP64 oracle's analytic gradient, which
     is itself checked against central finite differences in the CPU tests.
     n = 4099 takes the split-K path of the weight gradients (ragged last split)."""
-    net = Mlp(ctx, NIN, WIDTH, NH, NOUT, seed=5)
+    net = Mlp(ctx, NIN, WIDTH, NH, NOUT, seed=5, act=act)
     p = net.params()
     x, t = _data(n, seed=2)
     loss, g = net.loss_grad(x, t)
     rl, rg = oracle.mlp_loss_grad(p.astype(np.float64), x.astype(np.float64), t.astype(np.float64), NIN, WIDTH, NH,
-                                  NOUT, ACT_TANH)
+                                  NOUT, act)
     e = np.linalg.norm(g - rg) / np.linalg.norm(rg)
     print("loss", loss, rl, "grad rel err", e)
     assert abs(loss - rl) < 1e-5 * abs(rl)

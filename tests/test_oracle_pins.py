@@ -410,14 +410,14 @@ def test_pse_eps():
 # ---------------------------------------------------- far-field surrogate pins
 def test_mlp_oracle_gradient_matches_finite_differences():
     """The FP64 MLP oracle's analytic gradient against central differences
-    (both activations), on a small network and batch."""
+    (ReLU, tanh, ISRU), on a small network and batch."""
     rng = np.random.default_rng(21)
     nin, width, nh, nout = 5, 7, 3, 2
     npar = O.mlp_param_count(nin, width, nh, nout)
     p = rng.normal(scale=0.5, size=npar)
     x = rng.normal(size=(6, nin))
     t = rng.normal(size=(6, nout))
-    for act in (0, 1):
+    for act in (0, 1, 2):
         loss, g = O.mlp_loss_grad(p, x, t, nin, width, nh, nout, act)
         fd = np.zeros(npar)
         h = 1e-6
@@ -428,8 +428,22 @@ def test_mlp_oracle_gradient_matches_finite_differences():
             pp[k] -= 2 * h
             lm, _ = O.mlp_loss_grad(pp, x, t, nin, width, nh, nout, act)
             fd[k] = (lp - lm) / (2 * h)
-        tol = 1e-6 if act == 1 else 1e-4  # ReLU kinks
+        tol = 1e-4 if act == 0 else 1e-6  # ReLU kinks
         assert np.abs(fd - g).max() < tol * max(1.0, np.abs(g).max()), act
+
+
+def test_mlp_oracle_isru_by_hand():
+    """ISRU a / sqrt(1 + a^2) written out by hand on a one-hidden-layer network."""
+    W1 = np.array([[0.5, -1.0], [2.0, 0.25], [0.0, 1.0]])
+    b1 = np.array([0.1, -0.2, 0.3])
+    W2 = np.array([[1.0, -2.0, 0.5]])
+    b2 = np.array([0.7])
+    p = np.concatenate([W1.ravel(), b1, W2.ravel(), b2])
+    x = np.array([[0.3, -0.4], [1.0, 2.0], [-3.0, 5.0]])
+    a = x @ W1.T + b1
+    want = (a / np.sqrt(1 + a * a)) @ W2.T + b2
+    got = O.mlp_forward(p, x, 2, 3, 1, 1, 2)
+    assert np.allclose(got, want, rtol=1e-14, atol=1e-14)
 
 
 def test_mlp_oracle_forward_by_hand():
