@@ -543,8 +543,9 @@ struct NearArgs {
   const int2* items;            // (target cell, wave tile)
   const int* n_items;  // device count
   int ncell;
-  double* out;                  // [3][nt] FP64, cell order
+  double* out;                  // [split][3][nt] FP64, cell order
   int nt;
+  int nsplit;                   // near lists are split in nsplit parts (grid y)
 };
 
 __global__ __launch_bounds__(256) void ff_near_kernel(NearArgs a) {
@@ -572,8 +573,12 @@ __global__ __launch_bounds__(256) void ff_near_kernel(NearArgs a) {
   for (int r = 0; r < kR; ++r)
 #pragma unroll
     for (int c = 0; c < 3; ++c) S[r][c] = C[r][c] = 0.f;
+  // this split's share of the near list
   const int nn = a.near_count[t];
-  for (int q = 0; q < nn; ++q) {
+  const int split = blockIdx.y;
+  const int qb = (int)((long long)nn * split / a.nsplit), qe = (int)((long long)nn * (split + 1) / a.nsplit);
+  double* out = a.out + (size_t)split * 3 * a.nt;
+  for (int q = qb; q < qe; ++q) {
     const int s = __builtin_amdgcn_readfirstlane(a.near_list[(size_t)t * a.ncell + q]);
     const int j0 = a.src_start[s], j1 = a.src_start[s + 1];
     const bool checked = (s == t);  // a target's own source lives in its own cell
@@ -628,7 +633,7 @@ __global__ __launch_bounds__(256) void ff_near_kernel(NearArgs a) {
     const int k = k0 + 64 * r + lane;
     if (k < k_end)
 #pragma unroll
-      for (int c = 0; c < 3; ++c) a.out[(size_t)c * a.nt + k] = (double)S[r][c] + (double)C[r][c];
+      for (int c = 0; c < 3; ++c) out[(size_t)c * a.nt + k] = (double)S[r][c] + (double)C[r][c];
   }
 }
 
@@ -641,7 +646,9 @@ struct FarArgs {
   const int2* items;
   const int* n_items;  // device count
   int ncell, nt;
-  double* out;                // [3][nt], accumulated into (+=)
+  const double* near_part;    // [nsplit][3][nt] near-field partials
+  int nsplit;
+  double* out;                // [3][nt] = near partials (fixed order) + far
 };
 
 template <int P>
@@ -700,7 +707,11 @@ __global__ __launch_bounds__(256) void ff_far_kernel(FarArgs a) {
     const int k = k0 + 64 * r + lane;
     if (k < k_end)
 #pragma unroll
-      for (int c = 0; c < 3; ++c) a.out[(size_t)c * a.nt + k] += (double)S[r][c] + (double)C[r][c];
+      for (int c = 0; c < 3; ++c) {
+        double v = 0.0;
+        for (int sp = 0; sp < a.nsplit; ++sp) v += a.near_part[((size_t)sp * 3 + c) * a.nt + k];
+        a.out[(size_t)c * a.nt + k] = v + ((double)S[r][c] + (double)C[r][c]);
+      }
   }
 }
 
@@ -1029,6 +1040,8 @@ struct lpm_bve_ff {
   DevBuf cls, near_list, near_count, band_list, band_count;
   DevBuf items, mlp_items;
   DevBuf V;
+  DevBuf Vnear;        // near-field partials [near_split][3][n]
+  int near_split = 1;  // near lists split in this many parts (fills the chip)
   DevBuf list_plain;  // ascending active list for the plain path
   DevBuf tgt_plain, src_plain, partial_plain;
   DevBuf ptrs, err_part;
@@ -1119,16 +1132,17 @@ int ff_eval(lpm_bve_ff* f, int mode, const double* x, const double* y, const dou
   hipLaunchKernelGGL(ff_classify, dim3(nc), b256, 0, ctx->stream, ca);
   NearArgs na{f->trec.as<lpm::Vec4<float>>(), f->tgt_perm.as<int>(), f->tgt_start.as<int>(), f->srec.as<float>(),
               f->src_start.as<int>(), f->near_list.as<int>(), f->near_count.as<int>(), f->items.as<int2>(),
-              f->item_count.as<int>(), nc, f->V.as<double>(), f->n};
+              f->item_count.as<int>(), nc, f->Vnear.as<double>(), f->n, f->near_split};
   hipEvent_t e0 = nullptr, e1 = nullptr;
   if (ctx->timing) {
     e0 = lpm::take_event(ctx);
     e1 = lpm::take_event(ctx);
     if (e0 && e1) LPM_HIP(ctx, hipEventRecord(e0, ctx->stream));
   }
-  hipLaunchKernelGGL(ff_near_kernel, dim3((f->n_items + 3) / 4), b256, 0, ctx->stream, na);
+  hipLaunchKernelGGL(ff_near_kernel, dim3((f->n_items + 3) / 4, f->near_split), b256, 0, ctx->stream, na);
   FarArgs fa{f->trec.as<lpm::Vec4<float>>(), f->tgt_start.as<int>(), f->far_rec.as<float>(), f->cls.as<unsigned char>(),
-             f->items.as<int2>(), f->item_count.as<int>(), nc, f->n, f->V.as<double>()};
+             f->items.as<int2>(), f->item_count.as<int>(), nc, f->n, f->Vnear.as<double>(), f->near_split,
+             f->V.as<double>()};
   const dim3 gfar((f->n_items + 3) / 4);
   if (f->order == 2) hipLaunchKernelGGL(ff_far_kernel<2>, gfar, b256, 0, ctx->stream, fa);
   else if (f->order == 3) hipLaunchKernelGGL(ff_far_kernel<3>, gfar, b256, 0, ctx->stream, fa);
@@ -1279,6 +1293,17 @@ int lpm_bve_ff_new(lpm_ctx* ctx, lpm_bve_ff** out, int n, const double* x, const
   // work-item upper bounds: per cell ceil(count / tile) <= count / tile + 1
   f->n_items = ncell + n / (64 * kR) + 1;
   f->n_mlp_items = ncell + n / kMlpTile + 1;
+  // near-list splits: about three rounds of the near kernel's resident waves
+  {
+    int nb = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, reinterpret_cast<const void*>(ff_near_kernel), 256, 0) !=
+            hipSuccess ||
+        nb <= 0)
+      nb = 1;
+    const double waves = (double)n / (64 * kR) + 0.5 * ncell;
+    const double slots = (double)nb * 4 * ctx->num_cus;
+    f->near_split = std::min(4, std::max(1, (int)std::ceil(3.0 * slots / std::max(waves, 1.0))));
+  }
   const int end_bit = 32 - __builtin_clz((unsigned)ncell);  // keys 0..ncell
   size_t sort_bytes = 0;
   hipcub::DeviceRadixSort::SortPairs(nullptr, sort_bytes, (const int*)nullptr, (int*)nullptr, (const int*)nullptr,
@@ -1324,7 +1349,8 @@ int lpm_bve_ff_new(lpm_ctx* ctx, lpm_bve_ff** out, int n, const double* x, const
       chk(ff_upload(ctx, f->near_list, nullptr, (size_t)ncell * ncell * 4)) &&
       chk(ff_upload(ctx, f->near_count, nullptr, (size_t)ncell * 4)) &&
       chk(ff_upload(ctx, f->band_list, nullptr, (size_t)ncell * ncell * 4)) &&
-      chk(ff_upload(ctx, f->band_count, nullptr, (size_t)ncell * 4)) && chk(ff_upload(ctx, f->V, nullptr, 3 * nb));
+      chk(ff_upload(ctx, f->band_count, nullptr, (size_t)ncell * 4)) && chk(ff_upload(ctx, f->V, nullptr, 3 * nb)) &&
+      chk(ff_upload(ctx, f->Vnear, nullptr, (size_t)f->near_split * 3 * nb));
   for (int s = 0; s < 4 && rc == LPM_OK; ++s)
     for (int c = 0; c < 4 && rc == LPM_OK; ++c) chk(ff_upload(ctx, f->k[s][c], nullptr, nb));
   if (rc == LPM_OK) {
@@ -1592,7 +1618,7 @@ int lpm_bve_ff_delete(lpm_bve_ff* f) {
                    &f->near_list, &f->near_count, &f->band_list, &f->band_count, &f->items, &f->mlp_items, &f->V,
                    &f->list_plain, &f->tgt_plain, &f->src_plain, &f->partial_plain, &f->ptrs, &f->err_part, &f->sx,
                    &f->sy, &f->sid, &f->src_key, &f->key_t, &f->key_s, &f->iota, &f->cellof, &f->active,
-                   &f->sort_tmp, &f->item_count, &f->gen, &f->adir};
+                   &f->sort_tmp, &f->item_count, &f->gen, &f->adir, &f->Vnear};
   for (DevBuf* b : all) b->release();
   for (auto& row : f->k)
     for (auto& b : row) b.release();
