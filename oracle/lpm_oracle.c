@@ -61,6 +61,16 @@ enum {
   MUT_BVE_ZETA_SOURCE = 11,  /* d(zeta)/dt uses u instead of w             */
   MUT_PSE_K2_COEF = 12,      /* 4/pi -> 2/pi in the 2nd-order kernel       */
   MUT_BVE_RADIUS = 13,       /* strength divided by R^2 instead of R        */
+  MUT_PSE_GRAD_SIGN = 14,    /* (x_j - x_i) instead of (x_i - x_j)          */
+  MUT_PSE_K1_COEF = 15,      /* 5 rho^4 -> 4 rho^4 in the derivative kernel */
+  MUT_PSE_DELTA_COEF = 16,   /* 2 rho^4 -> 3 rho^4 in the delta kernel      */
+  MUT_PSE_NO_PROJ = 17,      /* sphere gradient/divergence not projected    */
+  MUT_DDOT_CROSS = 18,       /* 2 uy vx -> uy vx in the double dot          */
+  MUT_DDOT_W_ROW = 19,       /* sphere double dot w row with w_i, not v_i   */
+  MUT_PARTIALS_SYM = 20,     /* mixed partial dxy instead of (dxy + dyx)/2  */
+  MUT_SWE_POT_SIGN = 21,     /* potential (divergence) velocity sign        */
+  MUT_SWE_DENOM2 = 22,       /* velocity-gradient denominator 2 pi r^4      */
+  MUT_SWE_LAP_EPS = 23,      /* surface Laplacian missing the outer 1/eps^2 */
 };
 
 /* --------------------------------------------------------------------------
@@ -198,6 +208,307 @@ void lpm_oracle_pse_sphere_laplacian(int n, const double *x, const double *y, co
       acc = acc + lpm_oracle_pse_kernel(8, dist / eps) / (eps * eps) * (f[j] - f[i]) * area[j] / (eps * eps);
     }
     lap[i] = acc;
+  }
+}
+
+/* --------------------------------------------------------------------------
+ * PSE first-derivative, interpolation and double-dot operators,
+ * PSEDirectSum.f90:121-418, 537-579.
+ *   delta kernel  (bivariateDeltaKernel8, :611-616)
+ *     Kd(rho) = (4 - 6 rho^2 + 2 rho^4 - rho^6 / 6) exp(-rho^2) / pi
+ *   derivative kernel (bivariateFirstDerivativeKernel8, :629-634)
+ *     K1(rho) = (-20 + 20 rho^2 - 5 rho^4 + rho^6 / 3) exp(-rho^2) / pi
+ * Distances: ChordDistance on the plane (z = 0), SphereDistance (great
+ * circle, R atan2(|a x b|, a.b), SphereGeometry.f90:107-125) on the sphere.
+ * ------------------------------------------------------------------------ */
+double lpm_oracle_pse_delta_kernel(double rho) {
+  double r2 = rho * rho;
+  double c4 = (lpm_oracle_mutation == MUT_PSE_DELTA_COEF) ? 3.0 : 2.0;
+  return (4.0 - 6.0 * r2 + c4 * r2 * r2 - r2 * r2 * r2 / 6.0) * exp(-r2) / M_PI;
+}
+
+double lpm_oracle_pse_deriv_kernel(double rho) {
+  double r2 = rho * rho;
+  double c4 = (lpm_oracle_mutation == MUT_PSE_K1_COEF) ? 4.0 : 5.0;
+  return (-20.0 + 20.0 * r2 - c4 * r2 * r2 + r2 * r2 * r2 / 3.0) * exp(-r2) / M_PI;
+}
+
+static double sphere_distance(double ax, double ay, double az, double bx, double by, double bz,
+                              double radius) {
+  double cx = ay * bz - by * az;
+  double cy = bx * az - ax * bz;
+  double cz = ax * by - bx * ay;
+  return atan2(sqrt(cx * cx + cy * cy + cz * cz), ax * bx + ay * by + az * bz) * radius;
+}
+
+/* x_i - x_j, or its negation under the planted sign mistake. */
+static inline double sep(double a, double b) {
+  return (lpm_oracle_mutation == MUT_PSE_GRAD_SIGN) ? (b - a) : (a - b);
+}
+
+/* Rows of SphereProjection(x) = I - x x^T (SphereGeometry.f90:578-590)
+ * applied to g, in place; skipped under the planted mistake. */
+static void sphere_project(double xi, double yi, double zi, double g[3]) {
+  if (lpm_oracle_mutation == MUT_PSE_NO_PROJ) return;
+  double p0 = (1.0 - xi * xi) * g[0] - xi * yi * g[1] - xi * zi * g[2];
+  double p1 = -yi * xi * g[0] + (1.0 - yi * yi) * g[1] - yi * zi * g[2];
+  double p2 = -zi * xi * g[0] - zi * yi * g[1] + (1.0 - zi * zi) * g[2];
+  g[0] = p0;
+  g[1] = p1;
+  g[2] = p2;
+}
+
+/* PSEPlaneGradientAtParticles, PSEDirectSum.f90:180-217: inner factor
+ * K1/eps^3, whole field times 1/eps at the end (:216). */
+void lpm_oracle_pse_plane_gradient(int n, const double *x, const double *y, const double *f,
+                                   const double *area, const int *active, double eps, int i0,
+                                   int i1, double *gx, double *gy) {
+#pragma omp parallel for schedule(static)
+  for (int i = i0; i < i1; ++i) {
+    double ax = 0.0, ay = 0.0;
+    for (int j = 0; j < n; ++j) {
+      if (!active[j]) continue;
+      double dx = x[i] - x[j], dy = y[i] - y[j];
+      double kin = sqrt(dx * dx + dy * dy) / eps;
+      double k1 = lpm_oracle_pse_deriv_kernel(kin);
+      ax = ax + (f[j] + f[i]) * sep(x[i], x[j]) * k1 / (eps * eps * eps) * area[j];
+      ay = ay + (f[j] + f[i]) * sep(y[i], y[j]) * k1 / (eps * eps * eps) * area[j];
+    }
+    gx[i] = ax / eps;
+    gy[i] = ay / eps;
+  }
+}
+
+/* PSESphereGradientAtParticles, PSEDirectSum.f90:220-267: K1/eps^2 of the
+ * great-circle distance, projected by SphereProjection(x_i), times 1/eps^2. */
+void lpm_oracle_pse_sphere_gradient(int n, const double *x, const double *y, const double *z,
+                                    const double *f, const double *area, const int *active,
+                                    double eps, double radius, int i0, int i1, double *gx,
+                                    double *gy, double *gz) {
+#pragma omp parallel for schedule(static)
+  for (int i = i0; i < i1; ++i) {
+    double g[3] = {0.0, 0.0, 0.0};
+    for (int j = 0; j < n; ++j) {
+      if (!active[j]) continue;
+      double kin = sphere_distance(x[j], y[j], z[j], x[i], y[i], z[i], radius) / eps;
+      double kout = lpm_oracle_pse_deriv_kernel(kin) / (eps * eps);
+      g[0] = g[0] + (f[j] + f[i]) * sep(x[i], x[j]) * kout * area[j];
+      g[1] = g[1] + (f[j] + f[i]) * sep(y[i], y[j]) * kout * area[j];
+      g[2] = g[2] + (f[j] + f[i]) * sep(z[i], z[j]) * kout * area[j];
+    }
+    sphere_project(x[i], y[i], z[i], g);
+    gx[i] = g[0] / (eps * eps);
+    gy[i] = g[1] / (eps * eps);
+    gz[i] = g[2] / (eps * eps);
+  }
+}
+
+/* PSEPlaneSecondPartialsAtParticles, PSEDirectSum.f90:269-320: the PSE
+ * gradient of each component of a gradient field; outputs dxx,
+ * (dxy + dyx) / 2, dyy (xComp, yComp, zComp of the reference's Field). */
+void lpm_oracle_pse_plane_second_partials(int n, const double *x, const double *y,
+                                          const double *gx, const double *gy, const double *area,
+                                          const int *active, double eps, int i0, int i1,
+                                          double *dxx, double *dxy, double *dyy) {
+#pragma omp parallel for schedule(static)
+  for (int i = i0; i < i1; ++i) {
+    double axx = 0.0, axy = 0.0, ayx = 0.0, ayy = 0.0;
+    for (int j = 0; j < n; ++j) {
+      if (!active[j]) continue;
+      double dx = x[i] - x[j], dy = y[i] - y[j];
+      double kin = sqrt(dx * dx + dy * dy) / eps;
+      double kout = lpm_oracle_pse_deriv_kernel(kin) / (eps * eps);
+      axx = axx + (gx[j] + gx[i]) * sep(x[i], x[j]) / eps * kout * area[j];
+      axy = axy + (gx[j] + gx[i]) * sep(y[i], y[j]) / eps * kout * area[j];
+      ayx = ayx + (gy[j] + gy[i]) * sep(x[i], x[j]) / eps * kout * area[j];
+      ayy = ayy + (gy[j] + gy[i]) * sep(y[i], y[j]) / eps * kout * area[j];
+    }
+    dxx[i] = axx / eps;
+    dxy[i] = (lpm_oracle_mutation == MUT_PARTIALS_SYM ? axy : 0.5 * (axy + ayx)) / eps;
+    dyy[i] = ayy / eps;
+  }
+}
+
+/* PSEPlaneDoubleDotProductAtParticles, PSEDirectSum.f90:322-365:
+ * (ux^2 + 2 uy vx + vy^2) / eps^2 of the PSE velocity gradient. */
+void lpm_oracle_pse_plane_double_dot(int n, const double *x, const double *y, const double *u,
+                                     const double *v, const double *area, const int *active,
+                                     double eps, int i0, int i1, double *ddot) {
+  const double two = (lpm_oracle_mutation == MUT_DDOT_CROSS) ? 1.0 : 2.0;
+#pragma omp parallel for schedule(static)
+  for (int i = i0; i < i1; ++i) {
+    double ux = 0.0, uy = 0.0, vx = 0.0, vy = 0.0;
+    for (int j = 0; j < n; ++j) {
+      if (!active[j]) continue;
+      double dx = x[i] - x[j], dy = y[i] - y[j];
+      double kin = sqrt(dx * dx + dy * dy) / eps;
+      double kout = lpm_oracle_pse_deriv_kernel(kin) / (eps * eps);
+      ux = ux + (u[j] + u[i]) * sep(x[i], x[j]) / eps * kout * area[j];
+      uy = uy + (u[j] + u[i]) * sep(y[i], y[j]) / eps * kout * area[j];
+      vx = vx + (v[j] + v[i]) * sep(x[i], x[j]) / eps * kout * area[j];
+      vy = vy + (v[j] + v[i]) * sep(y[i], y[j]) / eps * kout * area[j];
+    }
+    ddot[i] = (ux * ux + two * uy * vx + vy * vy) / (eps * eps);
+  }
+}
+
+/* PSESphereDoubleDotProductAtParticles, PSEDirectSum.f90:367-418.  As in
+ * the reference, the w row pairs w_j with v_i (zComp(j) + yComp(i),
+ * :401-406), and no projection is applied. */
+void lpm_oracle_pse_sphere_double_dot(int n, const double *x, const double *y, const double *z,
+                                      const double *u, const double *v, const double *w,
+                                      const double *area, const int *active, double eps,
+                                      double radius, int i0, int i1, double *ddot) {
+  const double two = (lpm_oracle_mutation == MUT_DDOT_CROSS) ? 1.0 : 2.0;
+#pragma omp parallel for schedule(static)
+  for (int i = i0; i < i1; ++i) {
+    const double wi = (lpm_oracle_mutation == MUT_DDOT_W_ROW) ? w[i] : v[i];
+    double ux = 0.0, uy = 0.0, uz = 0.0, vx = 0.0, vy = 0.0, vz = 0.0, wx = 0.0, wy = 0.0,
+           wz = 0.0;
+    for (int j = 0; j < n; ++j) {
+      if (!active[j]) continue;
+      double kin = sphere_distance(x[i], y[i], z[i], x[j], y[j], z[j], radius) / eps;
+      double kout = lpm_oracle_pse_deriv_kernel(kin) / (eps * eps);
+      double sx = sep(x[i], x[j]) / eps, sy = sep(y[i], y[j]) / eps, sz = sep(z[i], z[j]) / eps;
+      ux = ux + (u[j] + u[i]) * sx * kout * area[j];
+      uy = uy + (u[j] + u[i]) * sy * kout * area[j];
+      uz = uz + (u[j] + u[i]) * sz * kout * area[j];
+      vx = vx + (v[j] + v[i]) * sx * kout * area[j];
+      vy = vy + (v[j] + v[i]) * sy * kout * area[j];
+      vz = vz + (v[j] + v[i]) * sz * kout * area[j];
+      wx = wx + (w[j] + wi) * sx * kout * area[j];
+      wy = wy + (w[j] + wi) * sy * kout * area[j];
+      wz = wz + (w[j] + wi) * sz * kout * area[j];
+    }
+    ddot[i] = (ux * ux + vy * vy + wz * wz + two * (uy * vx + uz * wx + vz * wy)) / (eps * eps);
+  }
+}
+
+/* PSESphereDivergenceAtParticles, PSEDirectSum.f90:537-579: per pair the
+ * projected kernel gradient P(x_i) (x_i - x_j) K1 / eps^3 dotted with
+ * (v_j + v_i) A_j; the field times 1/eps at the end. */
+void lpm_oracle_pse_sphere_divergence(int n, const double *x, const double *y, const double *z,
+                                      const double *u, const double *v, const double *w,
+                                      const double *area, const int *active, double eps,
+                                      double radius, int i0, int i1, double *div) {
+  const double denom = eps * eps * eps;
+#pragma omp parallel for schedule(static)
+  for (int i = i0; i < i1; ++i) {
+    double acc = 0.0;
+    for (int j = 0; j < n; ++j) {
+      if (!active[j]) continue;
+      double kin = sphere_distance(x[j], y[j], z[j], x[i], y[i], z[i], radius) / eps;
+      double k1 = lpm_oracle_pse_deriv_kernel(kin);
+      double g[3] = {sep(x[i], x[j]) * k1 / denom, sep(y[i], y[j]) * k1 / denom,
+                     sep(z[i], z[j]) * k1 / denom};
+      sphere_project(x[i], y[i], z[i], g);
+      acc = acc + (g[0] * (u[j] + u[i]) + g[1] * (v[j] + v[i]) + g[2] * (w[j] + w[i])) * area[j];
+    }
+    div[i] = acc / eps;
+  }
+}
+
+/* PSEPlaneInterpolateScalar, PSEDirectSum.f90:128-147, at each of nt points:
+ * sum over active j of f_j A_j Kd(|p - x_j| / eps) / eps^2 (no self term to
+ * skip: the points are not particles). */
+void lpm_oracle_pse_plane_interpolate(int n, const double *x, const double *y, const double *f,
+                                      const double *area, const int *active, double eps, int nt,
+                                      const double *px, const double *py, double *out) {
+#pragma omp parallel for schedule(static)
+  for (int k = 0; k < nt; ++k) {
+    double acc = 0.0;
+    for (int j = 0; j < n; ++j) {
+      if (!active[j]) continue;
+      double dx = px[k] - x[j], dy = py[k] - y[j];
+      double kin = sqrt(dx * dx + dy * dy) / eps;
+      acc = acc + f[j] * area[j] * lpm_oracle_pse_delta_kernel(kin) / (eps * eps);
+    }
+    out[k] = acc;
+  }
+}
+
+/* PSESphereInterpolateScalar, PSEDirectSum.f90:149-168, at each of nt points
+ * (great-circle distance). */
+void lpm_oracle_pse_sphere_interpolate(int n, const double *x, const double *y, const double *z,
+                                       const double *f, const double *area, const int *active,
+                                       double eps, double radius, int nt, const double *px,
+                                       const double *py, const double *pz, double *out) {
+#pragma omp parallel for schedule(static)
+  for (int k = 0; k < nt; ++k) {
+    double acc = 0.0;
+    for (int j = 0; j < n; ++j) {
+      if (!active[j]) continue;
+      double kin = sphere_distance(x[j], y[j], z[j], px[k], py[k], pz[k], radius) / eps;
+      acc = acc + f[j] * area[j] * lpm_oracle_pse_delta_kernel(kin) / (eps * eps);
+    }
+    out[k] = acc;
+  }
+}
+
+/* Sphere PSE Laplacian at nt points with given point values pf: the inline
+ * lat-lon grid loop of SpherePSEConvTest.f90:187-207,
+ *   lap(p) = sum_j K8(d(x_j, p) / eps) / eps^2 (f_j - pf) A_j / eps^2. */
+void lpm_oracle_pse_sphere_laplacian_points(int n, const double *x, const double *y,
+                                            const double *z, const double *f, const double *area,
+                                            const int *active, double eps, double radius, int nt,
+                                            const double *px, const double *py, const double *pz,
+                                            const double *pf, double *out) {
+#pragma omp parallel for schedule(static)
+  for (int k = 0; k < nt; ++k) {
+    double acc = 0.0;
+    for (int j = 0; j < n; ++j) {
+      if (!active[j]) continue;
+      double kin = sphere_distance(x[j], y[j], z[j], px[k], py[k], pz[k], radius) / eps;
+      acc = acc + lpm_oracle_pse_kernel(8, kin) / (eps * eps) * (f[j] - pf[k]) * area[j] / (eps * eps);
+    }
+    out[k] = acc;
+  }
+}
+
+/* --------------------------------------------------------------------------
+ * Planar shallow-water right-hand-side integrals, SWEPlaneRHSIntegrals,
+ * SWEPlaneSolver.f90:457-538.  bottom[j] holds topoFn(x_j, y_j), so the
+ * surface height is h_j + bottom_j.  For each target i:
+ *   lapSurf: sum over active j (j = i included, its term is 0) of
+ *            K8(r / eps) / eps^2 (s_j - s_i) A_j, then / eps^2;
+ *   u, v and the velocity gradient (ux, uy, vx, vy) of the point vortex /
+ *   point source field, sum over active j != i;
+ *   doubleDot = ux^2 + 2 uy vx + vy^2.
+ * ------------------------------------------------------------------------ */
+void lpm_oracle_swe_plane_rhs(int n, const double *x, const double *y, const double *vort,
+                              const double *divg, const double *h, const double *bottom,
+                              const double *area, const int *active, double eps, int i0, int i1,
+                              double *u, double *v, double *ddot, double *lap) {
+  const double psgn = (lpm_oracle_mutation == MUT_SWE_POT_SIGN) ? -1.0 : 1.0;
+  const double d2 = (lpm_oracle_mutation == MUT_SWE_DENOM2) ? 2.0 : 1.0;
+  const double outer = (lpm_oracle_mutation == MUT_SWE_LAP_EPS) ? 1.0 : 1.0 / (eps * eps);
+#pragma omp parallel for schedule(static)
+  for (int i = i0; i < i1; ++i) {
+    double ui = 0.0, vi = 0.0, li = 0.0, ux = 0.0, uy = 0.0, vx = 0.0, vy = 0.0;
+    const double si = h[i] + bottom[i];
+    for (int j = 0; j < n; ++j) {
+      if (!active[j]) continue;
+      double sj = h[j] + bottom[j];
+      double dx = x[i] - x[j], dy = y[i] - y[j];
+      double sq = dx * dx + dy * dy;
+      double kin = sqrt(sq) / eps;
+      li = li + lpm_oracle_pse_kernel(8, kin) / (eps * eps) * (sj - si) * area[j];
+      if (j == i) continue;
+      double denom = 2.0 * M_PI * sq;
+      double denom2 = d2 * M_PI * sq * sq;
+      double rot = vort[j] * area[j] / denom;
+      double pot = psgn * divg[j] * area[j] / denom;
+      ui = ui - dy * rot + dx * pot;
+      vi = vi + dx * rot + dy * pot;
+      ux = ux + pot - (dx * (dx * divg[j] - dy * vort[j])) * area[j] / denom2;
+      uy = uy - rot - (dy * (dx * divg[j] - dy * vort[j])) * area[j] / denom2;
+      vx = vx + rot - (dx * (dy * divg[j] + dx * vort[j])) * area[j] / denom2;
+      vy = vy + pot - (dy * (dy * divg[j] + dx * vort[j])) * area[j] / denom2;
+    }
+    u[i] = ui;
+    v[i] = vi;
+    lap[i] = li * outer;
+    ddot[i] = ux * ux + 2.0 * uy * vx + vy * vy;
   }
 }
 

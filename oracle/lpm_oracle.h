@@ -39,6 +39,60 @@ void lpm_oracle_pse_sphere_laplacian(int n, const double *x, const double *y, co
                                      const double *f, const double *area, const int *active,
                                      double eps, double radius, int i0, int i1, double *lap);
 
+/* PSE first-derivative, interpolation and double-dot operators
+ * (PSEDirectSum.f90:121-418, 537-579) and the planar SWE right-hand side
+ * (SWEPlaneSolver.f90:457-538). */
+double lpm_oracle_pse_delta_kernel(double rho);
+double lpm_oracle_pse_deriv_kernel(double rho);
+
+void lpm_oracle_pse_plane_gradient(int n, const double *x, const double *y, const double *f,
+                                   const double *area, const int *active, double eps, int i0,
+                                   int i1, double *gx, double *gy);
+
+void lpm_oracle_pse_sphere_gradient(int n, const double *x, const double *y, const double *z,
+                                    const double *f, const double *area, const int *active,
+                                    double eps, double radius, int i0, int i1, double *gx,
+                                    double *gy, double *gz);
+
+void lpm_oracle_pse_plane_second_partials(int n, const double *x, const double *y,
+                                          const double *gx, const double *gy, const double *area,
+                                          const int *active, double eps, int i0, int i1,
+                                          double *dxx, double *dxy, double *dyy);
+
+void lpm_oracle_pse_plane_double_dot(int n, const double *x, const double *y, const double *u,
+                                     const double *v, const double *area, const int *active,
+                                     double eps, int i0, int i1, double *ddot);
+
+void lpm_oracle_pse_sphere_double_dot(int n, const double *x, const double *y, const double *z,
+                                      const double *u, const double *v, const double *w,
+                                      const double *area, const int *active, double eps,
+                                      double radius, int i0, int i1, double *ddot);
+
+void lpm_oracle_pse_sphere_divergence(int n, const double *x, const double *y, const double *z,
+                                      const double *u, const double *v, const double *w,
+                                      const double *area, const int *active, double eps,
+                                      double radius, int i0, int i1, double *div);
+
+void lpm_oracle_pse_plane_interpolate(int n, const double *x, const double *y, const double *f,
+                                      const double *area, const int *active, double eps, int nt,
+                                      const double *px, const double *py, double *out);
+
+void lpm_oracle_pse_sphere_interpolate(int n, const double *x, const double *y, const double *z,
+                                       const double *f, const double *area, const int *active,
+                                       double eps, double radius, int nt, const double *px,
+                                       const double *py, const double *pz, double *out);
+
+void lpm_oracle_pse_sphere_laplacian_points(int n, const double *x, const double *y,
+                                            const double *z, const double *f, const double *area,
+                                            const int *active, double eps, double radius, int nt,
+                                            const double *px, const double *py, const double *pz,
+                                            const double *pf, double *out);
+
+void lpm_oracle_swe_plane_rhs(int n, const double *x, const double *y, const double *vort,
+                              const double *divg, const double *h, const double *bottom,
+                              const double *area, const int *active, double eps, int i0, int i1,
+                              double *u, double *v, double *ddot, double *lap);
+
 void lpm_oracle_bve_sphere_stream(int n, const double *x, const double *y, const double *z,
                                   const double *relvort, const double *absvort, const double *area,
                                   const int *active, double radius, int i0, int i1,

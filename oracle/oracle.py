@@ -27,6 +27,21 @@ _SIGS = {
     "lpm_oracle_pse_plane_laplacian": ([_i, _pd, _pd, _pd, _pd, _pi, _d, _i, _i, _i, _pd], None),
     "lpm_oracle_pse_kernel": ([_i, _d], _d),
     "lpm_oracle_pse_sphere_laplacian": ([_i, _pd, _pd, _pd, _pd, _pd, _pi, _d, _d, _i, _i, _pd], None),
+    "lpm_oracle_pse_delta_kernel": ([_d], _d),
+    "lpm_oracle_pse_deriv_kernel": ([_d], _d),
+    "lpm_oracle_pse_plane_gradient": ([_i, _pd, _pd, _pd, _pd, _pi, _d, _i, _i, _pd, _pd], None),
+    "lpm_oracle_pse_sphere_gradient": ([_i, _pd, _pd, _pd, _pd, _pd, _pi, _d, _d, _i, _i, _pd, _pd, _pd], None),
+    "lpm_oracle_pse_plane_second_partials": ([_i, _pd, _pd, _pd, _pd, _pd, _pi, _d, _i, _i, _pd, _pd, _pd], None),
+    "lpm_oracle_pse_plane_double_dot": ([_i, _pd, _pd, _pd, _pd, _pd, _pi, _d, _i, _i, _pd], None),
+    "lpm_oracle_pse_sphere_double_dot": ([_i, _pd, _pd, _pd, _pd, _pd, _pd, _pd, _pi, _d, _d, _i, _i, _pd], None),
+    "lpm_oracle_pse_sphere_divergence": ([_i, _pd, _pd, _pd, _pd, _pd, _pd, _pd, _pi, _d, _d, _i, _i, _pd], None),
+    "lpm_oracle_pse_plane_interpolate": ([_i, _pd, _pd, _pd, _pd, _pi, _d, _i, _pd, _pd, _pd], None),
+    "lpm_oracle_pse_sphere_interpolate": ([_i, _pd, _pd, _pd, _pd, _pd, _pi, _d, _d, _i, _pd, _pd, _pd, _pd],
+                                          None),
+    "lpm_oracle_pse_sphere_laplacian_points": ([_i, _pd, _pd, _pd, _pd, _pd, _pi, _d, _d, _i, _pd, _pd, _pd, _pd,
+                                                _pd], None),
+    "lpm_oracle_swe_plane_rhs": ([_i, _pd, _pd, _pd, _pd, _pd, _pd, _pd, _pi, _d, _i, _i, _pd, _pd, _pd, _pd],
+                                 None),
     "lpm_oracle_bve_sphere_stream": ([_i, _pd, _pd, _pd, _pd, _pd, _pd, _pi, _d, _i, _i, _pd, _pd], None),
     "lpm_oracle_plane_stream": ([_i, _pd, _pd, _pd, _pd, _pi, _i, _i, _pd], None),
     "lpm_oracle_bve_timestep": ([_i, _pd, _pd, _pd, _pd, _pd, _pd, _pd, _pd, _pi, _d, _d, _d], None),
@@ -138,6 +153,115 @@ def pse_sphere_laplacian(x, y, z, f, area, active, eps, radius=1.0, i0=0, i1=Non
     lib().lpm_oracle_pse_sphere_laplacian(n, _p(x), _p(y), _p(z), _p(f), _p(a), _q(act), eps, radius, i0, i1,
                                           _p(lap))
     return lap
+
+
+def pse_delta_kernel(rho):
+    return lib().lpm_oracle_pse_delta_kernel(rho)
+
+
+def pse_deriv_kernel(rho):
+    return lib().lpm_oracle_pse_deriv_kernel(rho)
+
+
+def _block(x, i0, i1):
+    n = x.shape[0]
+    return n, (n if i1 is None else i1)
+
+
+def pse_plane_gradient(x, y, f, area, active, eps, i0=0, i1=None):
+    x, y, f, a = map(_f, (x, y, f, area))
+    act = _ii(active)
+    n, i1 = _block(x, i0, i1)
+    gx, gy = np.zeros(n), np.zeros(n)
+    lib().lpm_oracle_pse_plane_gradient(n, _p(x), _p(y), _p(f), _p(a), _q(act), eps, i0, i1, _p(gx), _p(gy))
+    return gx, gy
+
+
+def pse_sphere_gradient(x, y, z, f, area, active, eps, radius=1.0, i0=0, i1=None):
+    x, y, z, f, a = map(_f, (x, y, z, f, area))
+    act = _ii(active)
+    n, i1 = _block(x, i0, i1)
+    g = [np.zeros(n) for _ in range(3)]
+    lib().lpm_oracle_pse_sphere_gradient(n, _p(x), _p(y), _p(z), _p(f), _p(a), _q(act), eps, radius, i0, i1,
+                                         *map(_p, g))
+    return tuple(g)
+
+
+def pse_plane_second_partials(x, y, gx, gy, area, active, eps, i0=0, i1=None):
+    x, y, gx, gy, a = map(_f, (x, y, gx, gy, area))
+    act = _ii(active)
+    n, i1 = _block(x, i0, i1)
+    d = [np.zeros(n) for _ in range(3)]
+    lib().lpm_oracle_pse_plane_second_partials(n, _p(x), _p(y), _p(gx), _p(gy), _p(a), _q(act), eps, i0, i1,
+                                               *map(_p, d))
+    return tuple(d)
+
+
+def pse_plane_double_dot(x, y, u, v, area, active, eps, i0=0, i1=None):
+    x, y, u, v, a = map(_f, (x, y, u, v, area))
+    act = _ii(active)
+    n, i1 = _block(x, i0, i1)
+    dd = np.zeros(n)
+    lib().lpm_oracle_pse_plane_double_dot(n, _p(x), _p(y), _p(u), _p(v), _p(a), _q(act), eps, i0, i1, _p(dd))
+    return dd
+
+
+def pse_sphere_double_dot(x, y, z, u, v, w, area, active, eps, radius=1.0, i0=0, i1=None):
+    x, y, z, u, v, w, a = map(_f, (x, y, z, u, v, w, area))
+    act = _ii(active)
+    n, i1 = _block(x, i0, i1)
+    dd = np.zeros(n)
+    lib().lpm_oracle_pse_sphere_double_dot(n, _p(x), _p(y), _p(z), _p(u), _p(v), _p(w), _p(a), _q(act), eps,
+                                           radius, i0, i1, _p(dd))
+    return dd
+
+
+def pse_sphere_divergence(x, y, z, u, v, w, area, active, eps, radius=1.0, i0=0, i1=None):
+    x, y, z, u, v, w, a = map(_f, (x, y, z, u, v, w, area))
+    act = _ii(active)
+    n, i1 = _block(x, i0, i1)
+    dv = np.zeros(n)
+    lib().lpm_oracle_pse_sphere_divergence(n, _p(x), _p(y), _p(z), _p(u), _p(v), _p(w), _p(a), _q(act), eps,
+                                           radius, i0, i1, _p(dv))
+    return dv
+
+
+def pse_plane_interpolate(x, y, f, area, active, eps, px, py):
+    x, y, f, a, px, py = map(_f, (x, y, f, area, px, py))
+    act = _ii(active)
+    out = np.zeros(px.shape[0])
+    lib().lpm_oracle_pse_plane_interpolate(x.shape[0], _p(x), _p(y), _p(f), _p(a), _q(act), eps, px.shape[0],
+                                           _p(px), _p(py), _p(out))
+    return out
+
+
+def pse_sphere_interpolate(x, y, z, f, area, active, eps, px, py, pz, radius=1.0):
+    x, y, z, f, a, px, py, pz = map(_f, (x, y, z, f, area, px, py, pz))
+    act = _ii(active)
+    out = np.zeros(px.shape[0])
+    lib().lpm_oracle_pse_sphere_interpolate(x.shape[0], _p(x), _p(y), _p(z), _p(f), _p(a), _q(act), eps, radius,
+                                            px.shape[0], _p(px), _p(py), _p(pz), _p(out))
+    return out
+
+
+def pse_sphere_laplacian_points(x, y, z, f, area, active, eps, px, py, pz, pf, radius=1.0):
+    x, y, z, f, a, px, py, pz, pf = map(_f, (x, y, z, f, area, px, py, pz, pf))
+    act = _ii(active)
+    out = np.zeros(px.shape[0])
+    lib().lpm_oracle_pse_sphere_laplacian_points(x.shape[0], _p(x), _p(y), _p(z), _p(f), _p(a), _q(act), eps,
+                                                 radius, px.shape[0], _p(px), _p(py), _p(pz), _p(pf), _p(out))
+    return out
+
+
+def swe_plane_rhs(x, y, vort, div, h, bottom, area, active, eps, i0=0, i1=None):
+    """Returns (u, v, double_dot, lap_surf)."""
+    x, y, q, d, h, b, a = map(_f, (x, y, vort, div, h, bottom, area))
+    act = _ii(active)
+    n, i1 = _block(x, i0, i1)
+    out = [np.zeros(n) for _ in range(4)]
+    lib().lpm_oracle_swe_plane_rhs(n, _p(x), _p(y), _p(q), _p(d), _p(h), _p(b), _p(a), _q(act), eps, i0, i1,
+                                   *map(_p, out))
+    return tuple(out)
 
 
 def bve_sphere_stream(x, y, z, relvort, absvort, area, active, radius=1.0, i0=0, i1=None):
