@@ -50,7 +50,12 @@ extern "C" {
 #define LPM_K_PLANE_GREEN 5  /* planar stream function          */
 #define LPM_K_FAR_FIELD 6    /* learned far field: near + expansion + network */
 #define LPM_K_PSE_SPHERE8 7  /* sphere PSE Laplacian, 8th-order kernel */
-#define LPM_K_COUNT 8
+#define LPM_K_PSE_DERIV_PLANE 8   /* PSE derivative sums, plane (gradient, partials, double dot) */
+#define LPM_K_PSE_DERIV_SPHERE 9  /* PSE derivative sums, sphere (gradient, double dot, divergence) */
+#define LPM_K_PSE_INTERP_PLANE 10 /* PSE interpolation at points, plane */
+#define LPM_K_PSE_INTERP_SPHERE 11 /* PSE interpolation at points, sphere */
+#define LPM_K_SWE_PLANE 12        /* planar shallow-water right-hand side */
+#define LPM_K_COUNT 13
 
 typedef struct lpm_ctx lpm_ctx;
 
@@ -106,6 +111,97 @@ int lpm_pse_sphere_laplacian(lpm_ctx *ctx, double *lap, const double *x, const d
                              const double *z, const double *f, const double *area,
                              const int32_t *active, int n, int i0, int i1, double eps,
                              double radius, int precision);
+
+/* Sphere PSE Laplacian at nt arbitrary points p_k with given point values pf:
+ *   lap_k = eps^-4 sum_{active j} (f_j - pf_k) K8(d(p_k, x_j) / eps) A_j
+ * The lat-lon grid loop of the reference's spherePSETest
+ * (SpherePSEConvTest.f90:187-207); no self pair is skipped. */
+int lpm_pse_sphere_laplacian_at_points(lpm_ctx *ctx, double *lap, const double *px, const double *py,
+                                       const double *pz, const double *pf, int nt, const double *x,
+                                       const double *y, const double *z, const double *f,
+                                       const double *area, const int32_t *active, int n, double eps,
+                                       double radius, int precision);
+
+/* PSE first-derivative operators.  With the derivative kernel
+ *   K1(rho) = (-20 + 20 rho^2 - 5 rho^4 + rho^6 / 3) exp(-rho^2) / pi
+ * (bivariateFirstDerivativeKernel8, PSEDirectSum.f90:629-634) and
+ *   D[c]_i = eps^-4 sum_{active j} (c_j + c_i) (x_i - x_j) K1(d_ij / eps) A_j,
+ * d_ij the chord (plane) or great-circle (sphere) distance:
+ *
+ * Plane gradient.  Replaces PSEPlaneGradientAtParticles (PSEDirectSum.f90:180-217):
+ *   (gx, gy)_i = D[f]_i */
+int lpm_pse_plane_gradient(lpm_ctx *ctx, double *gx, double *gy, const double *x, const double *y,
+                           const double *f, const double *area, const int32_t *active, int n,
+                           int i0, int i1, double eps, int precision);
+
+/* Sphere gradient.  Replaces PSESphereGradientAtParticles (PSEDirectSum.f90:220-267):
+ *   g_i = (I - x_i x_i^T) D[f]_i
+ * (SphereProjection, SphereGeometry.f90:578-590, as written for the unit
+ * sphere).  The separation x_i - x_j is taken as R (d_i - d_j) of the unit
+ * directions, which is x_i - x_j for particles on the sphere. */
+int lpm_pse_sphere_gradient(lpm_ctx *ctx, double *gx, double *gy, double *gz, const double *x,
+                            const double *y, const double *z, const double *f, const double *area,
+                            const int32_t *active, int n, int i0, int i1, double eps, double radius,
+                            int precision);
+
+/* Plane second partials of a gradient field (gx, gy).  Replaces
+ * PSEPlaneSecondPartialsAtParticles (PSEDirectSum.f90:269-320):
+ *   dxx = D[gx]_x,  dxy = (D[gx]_y + D[gy]_x) / 2,  dyy = D[gy]_y */
+int lpm_pse_plane_second_partials(lpm_ctx *ctx, double *dxx, double *dxy, double *dyy,
+                                  const double *x, const double *y, const double *gx,
+                                  const double *gy, const double *area, const int32_t *active,
+                                  int n, int i0, int i1, double eps, int precision);
+
+/* Plane double dot grad u : grad u.  Replaces PSEPlaneDoubleDotProductAtParticles
+ * (PSEDirectSum.f90:322-365):  ux^2 + 2 uy vx + vy^2 with (ux, uy) = D[u],
+ * (vx, vy) = D[v]. */
+int lpm_pse_plane_double_dot(lpm_ctx *ctx, double *ddot, const double *x, const double *y,
+                             const double *u, const double *v, const double *area,
+                             const int32_t *active, int n, int i0, int i1, double eps,
+                             int precision);
+
+/* Sphere double dot.  Replaces PSESphereDoubleDotProductAtParticles
+ * (PSEDirectSum.f90:367-418): ux^2 + vy^2 + wz^2 + 2 (uy vx + uz wx + vz wy)
+ * with the unprojected sums D[u], D[v] and, as the reference writes it
+ * (zComp(j) + yComp(i), :401-406), the w row summing (w_j + v_i). */
+int lpm_pse_sphere_double_dot(lpm_ctx *ctx, double *ddot, const double *x, const double *y,
+                              const double *z, const double *u, const double *v, const double *w,
+                              const double *area, const int32_t *active, int n, int i0, int i1,
+                              double eps, double radius, int precision);
+
+/* Sphere divergence.  Replaces PSESphereDivergenceAtParticles (PSEDirectSum.f90:537-579):
+ *   div_i = sum_{a,b} P_ab(x_i) D[v_a]_b,  P = I - x_i x_i^T */
+int lpm_pse_sphere_divergence(lpm_ctx *ctx, double *div, const double *x, const double *y,
+                              const double *z, const double *u, const double *v, const double *w,
+                              const double *area, const int32_t *active, int n, int i0, int i1,
+                              double eps, double radius, int precision);
+
+/* PSE interpolation at nt points.  Replaces PSEPlaneInterpolateScalar and
+ * PSESphereInterpolateScalar (PSEDirectSum.f90:121-168), one point per call
+ * there:  out_k = eps^-2 sum_{active j} f_j A_j Kd(d(p_k, x_j) / eps),
+ *   Kd(rho) = (4 - 6 rho^2 + 2 rho^4 - rho^6 / 6) exp(-rho^2) / pi
+ * (bivariateDeltaKernel8, PSEDirectSum.f90:611-616).  No self pair is skipped. */
+int lpm_pse_plane_interpolate(lpm_ctx *ctx, double *out, const double *px, const double *py, int nt,
+                              const double *x, const double *y, const double *f, const double *area,
+                              const int32_t *active, int n, double eps, int precision);
+int lpm_pse_sphere_interpolate(lpm_ctx *ctx, double *out, const double *px, const double *py,
+                               const double *pz, int nt, const double *x, const double *y,
+                               const double *z, const double *f, const double *area,
+                               const int32_t *active, int n, double eps, double radius,
+                               int precision);
+
+/* Planar shallow-water right-hand-side integrals.  Replaces SWEPlaneRHSIntegrals
+ * (SWEPlaneSolver.f90:457-538).  bottom[j] = topoFn(x_j, y_j) evaluated by the
+ * caller (the reference passes the function; the surface height is
+ * h_j + bottom_j).  For each target i, over active j:
+ *   (u, v)_i  = sum_{j != i} point-vortex + point-source velocity of
+ *               (vort_j A_j, div_j A_j),
+ *   ddot_i    = ux^2 + 2 uy vx + vy^2 of its exact gradient,
+ *   lapsurf_i = eps^-4 sum_j K8(|x_i - x_j| / eps) (s_j - s_i) A_j. */
+int lpm_swe_plane_rhs(lpm_ctx *ctx, double *u, double *v, double *ddot, double *lapsurf,
+                      const double *x, const double *y, const double *vort, const double *div,
+                      const double *h, const double *bottom, const double *area,
+                      const int32_t *active, int n, int i0, int i1, double eps, int precision);
 
 /* Sphere stream functions.  Replaces SetStreamFunctionsOnMesh (SphereBVE.f90:445-485).
  *   psi_i = sum_{active j != i} -log(R^2 - x_i.x_j) / (4 pi) zeta_j A_j

@@ -24,7 +24,8 @@ LIB = os.path.join(HERE, "liblpm_direct.so")
 OBJDIR = os.path.join(HERE, "_build")
 FORTRAN_EXE = os.path.join(HERE, "fortran", "lpm_fortran_smoke")
 
-HIP_SOURCES = ["lpm_pair.hip", "lpm_api.hip", "lpm_solver.hip", "lpm_batch.hip", "lpm_mlp.hip", "lpm_ff.hip"]
+HIP_SOURCES = ["lpm_pair.hip", "lpm_api.hip", "lpm_solver.hip", "lpm_batch.hip", "lpm_mlp.hip", "lpm_ff.hip",
+               "lpm_pse_ops.hip"]
 EXTRA_SOURCES = []  # appended by later subsystems
 
 HIPCC_FLAGS = [

@@ -59,6 +59,91 @@ int download(lpm_ctx* ctx, int i0, int i1, std::initializer_list<double*> outs) 
   return lpm::collect_timing(ctx);
 }
 
+// Common body of the PSE operator and SWE entry points.  `inputs` (n doubles
+// each) go to ctx->in[0..) in order, `outputs` to ctx->out[0..); the caller's
+// `bind` maps them onto the pair problem (kind, fields, op).  Points, when
+// given (npts >= 0), are uploaded after the inputs and become the targets.
+struct OpsCall {
+  const char* name;
+  int n, i0, i1, precision;
+  double eps, radius;
+  const int32_t* active;
+  std::initializer_list<const double*> inputs;
+  std::initializer_list<double*> outputs;
+  int npts = -1;
+  std::initializer_list<const double*> points = {};
+};
+
+template <class Bind>
+int run_ops_call(lpm_ctx* ctx, const OpsCall& c, Bind bind) {
+  if (!ctx) return LPM_ERR_ARG;
+  ctx->err.clear();
+  const std::string nm = c.name;
+  const bool at_points = c.npts >= 0;
+  if (at_points) {
+    if (c.n < 0) return fail(ctx, LPM_ERR_ARG, nm + ": n must be >= 0");
+  } else if (int rc = check_block(ctx, c.n, c.i0, c.i1)) {
+    return rc;
+  }
+  if (int rc = check_prec(ctx, c.precision)) return rc;
+  bool null_arg = !c.active;
+  for (const double* a : c.inputs) null_arg = null_arg || !a;
+  for (const double* a : c.points) null_arg = null_arg || !a;
+  for (double* a : c.outputs) null_arg = null_arg || !a;
+  if (null_arg) return fail(ctx, LPM_ERR_ARG, nm + ": null array argument");
+  if (!(c.eps > 0.0) || !(c.radius > 0.0)) return fail(ctx, LPM_ERR_ARG, nm + ": eps and radius must be > 0");
+  LPM_HIP(ctx, hipSetDevice(ctx->device));
+  const int t0 = at_points ? 0 : c.i0, t1 = at_points ? c.npts : c.i1;
+  if (t1 == t0) return LPM_OK;
+  int ns = 0;
+  if (int rc = lpm::upload_active_list(ctx, c.active, c.n, ctx->list, nullptr, &ns)) return rc;
+  int k = 0;
+  for (const double* a : c.inputs) {
+    DevBuf& b = ctx->in[k++];
+    LPM_HIP(ctx, b.ensure((size_t)std::max(c.n, 1) * sizeof(double)));
+    if (c.n > 0)
+      LPM_HIP(ctx, hipMemcpyAsync(b.p, a, (size_t)c.n * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+  }
+  const int k_pts = k;
+  for (const double* a : c.points) {
+    DevBuf& b = ctx->in[k++];
+    LPM_HIP(ctx, b.ensure((size_t)std::max(c.npts, 1) * sizeof(double)));
+    LPM_HIP(ctx, hipMemcpyAsync(b.p, a, (size_t)c.npts * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+  }
+  const int nout_rows = at_points ? c.npts : c.n;
+  if (int rc = prepare_outputs(ctx, nout_rows, (int)c.outputs.size())) return rc;
+  lpm::PairProblem pp;
+  pp.precision = c.precision;
+  pp.n = c.n;
+  pp.i0 = t0;
+  pp.i1 = t1;
+  pp.src_list = ctx->list.as<int>();
+  pp.ns = ns;
+  pp.eps = c.eps;
+  pp.radius = c.radius;
+  pp.out0 = ctx->out[0].as<double>();
+  pp.out1 = ctx->out[1].as<double>();
+  pp.out2 = ctx->out[2].as<double>();
+  pp.out3 = ctx->out[3].as<double>();
+  auto in = [&](int i) { return ctx->in[i].as<double>(); };
+  if (at_points) {
+    pp.tx = in(k_pts);
+    pp.ty = in(k_pts + 1);
+    pp.tz = c.points.size() > 2 ? in(k_pts + 2) : nullptr;
+    pp.tq = c.points.size() > 3 ? in(k_pts + 3) : nullptr;
+  }
+  bind(pp, in);
+  if (int rc = lpm::run_pair_sum(ctx, pp)) return rc;
+  // outputs are copied back in the order given
+  int o = 0;
+  for (double* out : c.outputs) {
+    LPM_HIP(ctx, hipMemcpyAsync(out + t0, ctx->out[o++].as<double>() + t0, (size_t)(t1 - t0) * sizeof(double),
+                                hipMemcpyDeviceToHost, ctx->stream));
+  }
+  LPM_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return lpm::collect_timing(ctx);
+}
+
 }  // namespace
 
 extern "C" {
@@ -279,6 +364,149 @@ int lpm_pse_sphere_laplacian(lpm_ctx* ctx, double* lap, const double* x, const d
   pp.out0 = ctx->out[0].as<double>();
   if (int rc = lpm::run_pair_sum(ctx, pp)) return rc;
   return download(ctx, i0, i1, {lap});
+}
+
+int lpm_pse_sphere_laplacian_at_points(lpm_ctx* ctx, double* lap, const double* px, const double* py,
+                                       const double* pz, const double* pf, int nt, const double* x,
+                                       const double* y, const double* z, const double* f, const double* area,
+                                       const int32_t* active, int n, double eps, double radius, int precision) {
+  if (nt < 0) return ctx ? fail(ctx, LPM_ERR_ARG, "lpm_pse_sphere_laplacian_at_points: nt must be >= 0") : LPM_ERR_ARG;
+  return run_ops_call(ctx,
+                      OpsCall{"lpm_pse_sphere_laplacian_at_points", n, 0, 0, precision, eps, radius, active,
+                              {x, y, z, f, area}, {lap}, nt, {px, py, pz, pf}},
+                      [](lpm::PairProblem& pp, auto in) {
+                        pp.kind = LPM_K_PSE_SPHERE8;
+                        pp.x = in(0), pp.y = in(1), pp.z = in(2), pp.q0 = in(3), pp.area = in(4);
+                      });
+}
+
+int lpm_pse_plane_gradient(lpm_ctx* ctx, double* gx, double* gy, const double* x, const double* y,
+                           const double* f, const double* area, const int32_t* active, int n, int i0, int i1,
+                           double eps, int precision) {
+  return run_ops_call(ctx,
+                      OpsCall{"lpm_pse_plane_gradient", n, i0, i1, precision, eps, 1.0, active, {x, y, f, area},
+                              {gx, gy}},
+                      [](lpm::PairProblem& pp, auto in) {
+                        pp.kind = LPM_K_PSE_DERIV_PLANE;
+                        pp.op = lpm::kOpGradient;
+                        pp.m = 1;
+                        pp.x = in(0), pp.y = in(1), pp.q0 = in(2), pp.area = in(3);
+                      });
+}
+
+int lpm_pse_sphere_gradient(lpm_ctx* ctx, double* gx, double* gy, double* gz, const double* x, const double* y,
+                            const double* z, const double* f, const double* area, const int32_t* active, int n,
+                            int i0, int i1, double eps, double radius, int precision) {
+  return run_ops_call(ctx,
+                      OpsCall{"lpm_pse_sphere_gradient", n, i0, i1, precision, eps, radius, active,
+                              {x, y, z, f, area}, {gx, gy, gz}},
+                      [](lpm::PairProblem& pp, auto in) {
+                        pp.kind = LPM_K_PSE_DERIV_SPHERE;
+                        pp.op = lpm::kOpGradient;
+                        pp.m = 1;
+                        pp.x = in(0), pp.y = in(1), pp.z = in(2), pp.q0 = in(3), pp.area = in(4);
+                      });
+}
+
+int lpm_pse_plane_second_partials(lpm_ctx* ctx, double* dxx, double* dxy, double* dyy, const double* x,
+                                  const double* y, const double* gx, const double* gy, const double* area,
+                                  const int32_t* active, int n, int i0, int i1, double eps, int precision) {
+  return run_ops_call(ctx,
+                      OpsCall{"lpm_pse_plane_second_partials", n, i0, i1, precision, eps, 1.0, active,
+                              {x, y, gx, gy, area}, {dxx, dxy, dyy}},
+                      [](lpm::PairProblem& pp, auto in) {
+                        pp.kind = LPM_K_PSE_DERIV_PLANE;
+                        pp.op = lpm::kOpSecondPartials;
+                        pp.m = 2;
+                        pp.x = in(0), pp.y = in(1), pp.q0 = in(2), pp.q1 = in(3), pp.area = in(4);
+                      });
+}
+
+int lpm_pse_plane_double_dot(lpm_ctx* ctx, double* ddot, const double* x, const double* y, const double* u,
+                             const double* v, const double* area, const int32_t* active, int n, int i0, int i1,
+                             double eps, int precision) {
+  return run_ops_call(ctx,
+                      OpsCall{"lpm_pse_plane_double_dot", n, i0, i1, precision, eps, 1.0, active,
+                              {x, y, u, v, area}, {ddot}},
+                      [](lpm::PairProblem& pp, auto in) {
+                        pp.kind = LPM_K_PSE_DERIV_PLANE;
+                        pp.op = lpm::kOpDoubleDot;
+                        pp.m = 2;
+                        pp.x = in(0), pp.y = in(1), pp.q0 = in(2), pp.q1 = in(3), pp.area = in(4);
+                      });
+}
+
+int lpm_pse_sphere_double_dot(lpm_ctx* ctx, double* ddot, const double* x, const double* y, const double* z,
+                              const double* u, const double* v, const double* w, const double* area,
+                              const int32_t* active, int n, int i0, int i1, double eps, double radius,
+                              int precision) {
+  return run_ops_call(ctx,
+                      OpsCall{"lpm_pse_sphere_double_dot", n, i0, i1, precision, eps, radius, active,
+                              {x, y, z, u, v, w, area}, {ddot}},
+                      [](lpm::PairProblem& pp, auto in) {
+                        pp.kind = LPM_K_PSE_DERIV_SPHERE;
+                        pp.op = lpm::kOpDoubleDot;
+                        pp.m = 3;
+                        pp.x = in(0), pp.y = in(1), pp.z = in(2);
+                        pp.q0 = in(3), pp.q1 = in(4), pp.q2 = in(5), pp.area = in(6);
+                      });
+}
+
+int lpm_pse_sphere_divergence(lpm_ctx* ctx, double* div, const double* x, const double* y, const double* z,
+                              const double* u, const double* v, const double* w, const double* area,
+                              const int32_t* active, int n, int i0, int i1, double eps, double radius,
+                              int precision) {
+  return run_ops_call(ctx,
+                      OpsCall{"lpm_pse_sphere_divergence", n, i0, i1, precision, eps, radius, active,
+                              {x, y, z, u, v, w, area}, {div}},
+                      [](lpm::PairProblem& pp, auto in) {
+                        pp.kind = LPM_K_PSE_DERIV_SPHERE;
+                        pp.op = lpm::kOpDivergence;
+                        pp.m = 3;
+                        pp.x = in(0), pp.y = in(1), pp.z = in(2);
+                        pp.q0 = in(3), pp.q1 = in(4), pp.q2 = in(5), pp.area = in(6);
+                      });
+}
+
+int lpm_pse_plane_interpolate(lpm_ctx* ctx, double* out, const double* px, const double* py, int nt,
+                              const double* x, const double* y, const double* f, const double* area,
+                              const int32_t* active, int n, double eps, int precision) {
+  if (nt < 0) return ctx ? fail(ctx, LPM_ERR_ARG, "lpm_pse_plane_interpolate: nt must be >= 0") : LPM_ERR_ARG;
+  return run_ops_call(ctx,
+                      OpsCall{"lpm_pse_plane_interpolate", n, 0, 0, precision, eps, 1.0, active, {x, y, f, area},
+                              {out}, nt, {px, py}},
+                      [](lpm::PairProblem& pp, auto in) {
+                        pp.kind = LPM_K_PSE_INTERP_PLANE;
+                        pp.x = in(0), pp.y = in(1), pp.q0 = in(2), pp.area = in(3);
+                      });
+}
+
+int lpm_pse_sphere_interpolate(lpm_ctx* ctx, double* out, const double* px, const double* py, const double* pz,
+                               int nt, const double* x, const double* y, const double* z, const double* f,
+                               const double* area, const int32_t* active, int n, double eps, double radius,
+                               int precision) {
+  if (nt < 0) return ctx ? fail(ctx, LPM_ERR_ARG, "lpm_pse_sphere_interpolate: nt must be >= 0") : LPM_ERR_ARG;
+  return run_ops_call(ctx,
+                      OpsCall{"lpm_pse_sphere_interpolate", n, 0, 0, precision, eps, radius, active,
+                              {x, y, z, f, area}, {out}, nt, {px, py, pz}},
+                      [](lpm::PairProblem& pp, auto in) {
+                        pp.kind = LPM_K_PSE_INTERP_SPHERE;
+                        pp.x = in(0), pp.y = in(1), pp.z = in(2), pp.q0 = in(3), pp.area = in(4);
+                      });
+}
+
+int lpm_swe_plane_rhs(lpm_ctx* ctx, double* u, double* v, double* ddot, double* lapsurf, const double* x,
+                      const double* y, const double* vort, const double* div, const double* h,
+                      const double* bottom, const double* area, const int32_t* active, int n, int i0, int i1,
+                      double eps, int precision) {
+  return run_ops_call(ctx,
+                      OpsCall{"lpm_swe_plane_rhs", n, i0, i1, precision, eps, 1.0, active,
+                              {x, y, vort, div, h, bottom, area}, {u, v, ddot, lapsurf}},
+                      [](lpm::PairProblem& pp, auto in) {
+                        pp.kind = LPM_K_SWE_PLANE;
+                        pp.x = in(0), pp.y = in(1);
+                        pp.q0 = in(2), pp.q1 = in(3), pp.q2 = in(4), pp.q3 = in(5), pp.area = in(6);
+                      });
 }
 
 int lpm_bve_sphere_stream(lpm_ctx* ctx, double* psi_rel, double* psi_abs, const double* x,

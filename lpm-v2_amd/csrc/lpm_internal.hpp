@@ -59,8 +59,8 @@ struct lpm_ctx {
   hipStream_t stream = nullptr;
   std::string err;
   // stateless-call staging
-  lpm::DevBuf in[8];
-  lpm::DevBuf out[3];
+  lpm::DevBuf in[12];
+  lpm::DevBuf out[4];
   lpm::DevBuf list;
   lpm::DevBuf tgt, src, partial;
   lpm::DevBuf red;  // reductions
@@ -109,11 +109,28 @@ struct PairProblem {
   double* out0 = nullptr;  // device outputs, indexed by particle (i0..i1-1)
   double* out1 = nullptr;
   double* out2 = nullptr;
+  // PSE derivative / SWE kinds: more fields and outputs, the epilogue op and
+  // the number of derivative fields m.
+  const double* q2 = nullptr;
+  const double* q3 = nullptr;
+  double* out3 = nullptr;
+  int op = 0;
+  int m = 1;
+  // Targets at points other than the particles (interpolation, Laplacian at
+  // points): when tx is set, targets are tx/ty/tz[/tq][i0..i1), outputs are
+  // indexed by point, and no self pair is excluded.
+  const double* tx = nullptr;
+  const double* ty = nullptr;
+  const double* tz = nullptr;
+  const double* tq = nullptr;
   // Optional caller-owned scratch; when null the context's buffers are used.
   DevBuf* tgt_buf = nullptr;
   DevBuf* src_buf = nullptr;
   DevBuf* partial_buf = nullptr;
 };
+
+// Epilogues of the PSE derivative kinds.
+enum PseOp { kOpGradient = 0, kOpSecondPartials = 1, kOpDoubleDot = 2, kOpDivergence = 3 };
 
 // Enqueue the whole sum (pack, pair kernel, finalize) on ctx->stream.
 int run_pair_sum(lpm_ctx* ctx, const PairProblem& pp);
@@ -127,7 +144,35 @@ struct PackArgs {
   int count;  // records to write
   int i0;     // first target particle (targets)
   double radius, sigma;
+  const double* q2 = nullptr;
+  const double* q3 = nullptr;
+  int m = 1;        // derivative fields
+  double aux = 0.0; // SWE: log2(e)/eps^2 (FP32) or 1/eps^2 (FP64)
 };
+
+// Ordered sum over source splits and the FP64 epilogue of one kind.
+struct FinArgs {
+  const double* partial;
+  int nt, i0, nsplit;
+  const double *x, *y, *z, *q0, *q1, *area;
+  const int* active;
+  const double* G;
+  double radius, scale;
+  int fp32;
+  double *out0, *out1, *out2;
+  const double* q2 = nullptr;
+  double* out3 = nullptr;
+  int op = 0;
+};
+
+// The PSE derivative, interpolation and SWE kinds (lpm_pse_ops.hip).
+bool is_ops_kind(int kind);
+int ops_nout(int kind, int m);
+int ops_record_words(int kind, int m);
+const void* ops_pick_kernel(int kind, int prec, int r, int m, bool series);
+void ops_launch_pack(int kind, int prec, const PackArgs& s, const PackArgs& t, void* srec, void* trec, int ns,
+                     int nt, hipStream_t st);
+int ops_launch_finalize(int kind, int m, const FinArgs& f, hipStream_t st);
 void launch_pack(int kind, int precision, const PackArgs& a, void* srec, void* trec, int ns, int nt,
                  hipStream_t st);
 Tuning default_tuning(int kind, int prec);

@@ -169,17 +169,6 @@ __global__ __launch_bounds__(256) void source_totals(int ns, const int* __restri
 }
 
 // ---------------------------------------------------------------- finalize
-struct FinArgs {
-  const double* partial;
-  int nt, i0, nsplit;
-  const double *x, *y, *z, *q0, *q1, *area;
-  const int* active;
-  const double* G;
-  double radius, scale;
-  int fp32;
-  double *out0, *out1, *out2;
-};
-
 template <int KIND, int NO>
 __global__ __launch_bounds__(256) void finalize_kernel(FinArgs f) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -289,11 +278,11 @@ static KernelFn pick_kernel(int kind, int prec, int r, bool variant) {
 }
 
 template <typename T>
-static void launch_pack_t(int kind, const PackArgs& a, void* srec, void* trec, int ns, int nt,
-                          hipStream_t st) {
+static void launch_pack_t(int kind, const PackArgs& a_src, const PackArgs& a_tgt, void* srec, void* trec,
+                          int ns, int nt, hipStream_t st) {
   dim3 b(256);
   dim3 gs((ns + 255) / 256), gt((nt + 255) / 256);
-  PackArgs as = a, at = a;
+  PackArgs as = a_src, at = a_tgt;
   as.count = ns;
   at.count = nt;
 #define LPM_PACK_CASE(K)                                                                    \
@@ -313,12 +302,17 @@ static void launch_pack_t(int kind, const PackArgs& a, void* srec, void* trec, i
 #undef LPM_PACK_CASE
 }
 
+static void launch_pack2(int kind, int precision, const PackArgs& as, const PackArgs& at, void* srec,
+                         void* trec, int ns, int nt, hipStream_t st) {
+  if (precision == LPM_PREC_FP32_KAHAN)
+    launch_pack_t<float>(kind, as, at, srec, trec, ns, nt, st);
+  else
+    launch_pack_t<double>(kind, as, at, srec, trec, ns, nt, st);
+}
+
 void launch_pack(int kind, int precision, const PackArgs& a, void* srec, void* trec, int ns, int nt,
                  hipStream_t st) {
-  if (precision == LPM_PREC_FP32_KAHAN)
-    launch_pack_t<float>(kind, a, srec, trec, ns, nt, st);
-  else
-    launch_pack_t<double>(kind, a, srec, trec, ns, nt, st);
+  launch_pack2(kind, precision, a, a, srec, trec, ns, nt, st);
 }
 
 static void launch_finalize(int kind, const FinArgs& f, hipStream_t st) {
@@ -420,7 +414,16 @@ int run_pair_sum(lpm_ctx* ctx, const PairProblem& pp) {
   tu.splits = ov.splits;
 
   const size_t wsz = prec == LPM_PREC_FP32_KAHAN ? 4 : 8;
-  const int no = nout_of(kind);
+  const bool ops = is_ops_kind(kind);
+  const int no = ops ? ops_nout(kind, pp.m) : nout_of(kind);
+  const int nw = ops ? ops_record_words(kind, pp.m) : SREC;
+  if (ops && !ov.r) {
+    // up to 12 sums per target: fewer targets per lane keep the registers
+    if (prec == LPM_PREC_FP32_KAHAN)
+      tu.r = no > 4 ? 2 : 4;
+    else
+      tu.r = no > 4 ? 1 : 2;
+  }
   const int ns = pp.ns;
   DevBuf& tb = pp.tgt_buf ? *pp.tgt_buf : ctx->tgt;
   DevBuf& sb = pp.src_buf ? *pp.src_buf : ctx->src;
@@ -428,9 +431,13 @@ int run_pair_sum(lpm_ctx* ctx, const PairProblem& pp) {
 
   const int per_wave = 64 * tu.r;
   const int n_wtiles = (nt + per_wave - 1) / per_wave;
-  // sphere PSE in FP32: the series form of the distance when 7 eps <= 0.56 R
-  const bool variant = kind == LPM_K_PSE_SPHERE8 ? (7.0 * pp.eps <= 0.56 * pp.radius) : pp.on_sphere;
-  KernelFn fn = pick_kernel(kind, prec, tu.r, variant);
+  // sphere PSE kinds in FP32: the series form of the distance when 7 eps <= 0.56 R
+  const bool sphere_pse = kind == LPM_K_PSE_SPHERE8 || kind == LPM_K_PSE_DERIV_SPHERE ||
+                          kind == LPM_K_PSE_INTERP_SPHERE;
+  const bool variant = sphere_pse ? (7.0 * pp.eps <= 0.56 * pp.radius) : pp.on_sphere;
+  KernelFn fn = ops ? reinterpret_cast<KernelFn>(const_cast<void*>(ops_pick_kernel(kind, prec, tu.r, pp.m, variant)))
+                    : pick_kernel(kind, prec, tu.r, variant);
+  if (!fn) return fail(ctx, LPM_ERR_ARG, "no pair kernel for this kind, precision and field count");
   int nsplit = 1, chunk = std::max(ns, 1);
   if (ns > 0) {
     int want = tu.splits > 0 ? tu.splits
@@ -441,13 +448,30 @@ int run_pair_sum(lpm_ctx* ctx, const PairProblem& pp) {
   }
 
   LPM_HIP(ctx, tb.ensure((size_t)nt * TREC * wsz));
-  LPM_HIP(ctx, sb.ensure((size_t)std::max(ns, 1) * SREC * wsz));
+  LPM_HIP(ctx, sb.ensure((size_t)std::max(ns, 1) * nw * wsz));
   LPM_HIP(ctx, pb.ensure((size_t)nsplit * no * nt * sizeof(double)));
 
+  const bool fp32 = prec == LPM_PREC_FP32_KAHAN;
   PackArgs pa{pp.x, pp.y, pp.z, pp.q0, pp.q1, pp.area, nullptr, pp.src_list, 0, pp.i0, pp.radius, 1.0};
-  if (kind == LPM_K_PSE8 || kind == LPM_K_PSE2 || kind == LPM_K_PSE_SPHERE8)
-    pa.sigma = (prec == LPM_PREC_FP32_KAHAN ? std::sqrt(kLog2e) : 1.0) / pp.eps;
-  launch_pack(kind, prec, pa, sb.p, tb.p, ns, nt, ctx->stream);
+  pa.q2 = pp.q2;
+  pa.q3 = pp.q3;
+  pa.m = pp.m;
+  const bool scaled = kind == LPM_K_PSE8 || kind == LPM_K_PSE2 || kind == LPM_K_PSE_SPHERE8 ||
+                      kind == LPM_K_PSE_DERIV_PLANE || kind == LPM_K_PSE_DERIV_SPHERE ||
+                      kind == LPM_K_PSE_INTERP_PLANE || kind == LPM_K_PSE_INTERP_SPHERE;
+  if (scaled) pa.sigma = (fp32 ? std::sqrt(kLog2e) : 1.0) / pp.eps;
+  if (kind == LPM_K_SWE_PLANE) pa.aux = (fp32 ? kLog2e : 1.0) / (pp.eps * pp.eps);
+  PackArgs pt = pa;
+  if (pp.tx) {
+    pt.x = pp.tx;
+    pt.y = pp.ty;
+    pt.z = pp.tz;
+    pt.q0 = pp.tq;
+  }
+  if (ops)
+    ops_launch_pack(kind, prec, pa, pt, sb.p, tb.p, ns, nt, ctx->stream);
+  else
+    launch_pack2(kind, prec, pa, pt, sb.p, tb.p, ns, nt, ctx->stream);
   LPM_HIP(ctx, hipGetLastError());
 
   if (ns > 0) {
@@ -456,7 +480,8 @@ int run_pair_sum(lpm_ctx* ctx, const PairProblem& pp) {
     a.src = sb.p;
     a.partial = pb.as<double>();
     a.nt = nt;
-    a.tgt_idx0 = pp.i0;
+    // targets at points: an index no source has, so no pair is skipped
+    a.tgt_idx0 = pp.tx ? -(1 << 30) : pp.i0;
     a.ns = ns;
     a.chunk = chunk;
     a.nsplit = nsplit;
@@ -503,17 +528,28 @@ int run_pair_sum(lpm_ctx* ctx, const PairProblem& pp) {
   f.out0 = pp.out0;
   f.out1 = pp.out1;
   f.out2 = pp.out2;
-  if (kind == LPM_K_PSE8 || kind == LPM_K_PSE2 || kind == LPM_K_PSE_SPHERE8) {
-    const double e4 = pp.eps * pp.eps * pp.eps * pp.eps;
+  f.q2 = pp.q2;
+  f.out3 = pp.out3;
+  f.op = pp.op;
+  const double e4 = pp.eps * pp.eps * pp.eps * pp.eps;
+  if (kind == LPM_K_PSE8 || kind == LPM_K_PSE2 || kind == LPM_K_PSE_SPHERE8)
     f.scale = (kind == LPM_K_PSE2 ? 4.0 : 1.0) / (kPi * e4);
-  }
+  // derivative sums carry the separation scaled by sigma
+  if (kind == LPM_K_PSE_DERIV_PLANE || kind == LPM_K_PSE_DERIV_SPHERE) f.scale = 1.0 / (pa.sigma * kPi * e4);
+  if (kind == LPM_K_PSE_INTERP_PLANE || kind == LPM_K_PSE_INTERP_SPHERE) f.scale = 1.0 / (kPi * pp.eps * pp.eps);
+  if (kind == LPM_K_SWE_PLANE) f.scale = 1.0 / (kPi * e4);
   if (kind == LPM_K_SPHERE_GREEN && f.fp32) {
     LPM_HIP(ctx, ctx->red.ensure(2 * sizeof(double)));
     hipLaunchKernelGGL(source_totals, dim3(1), dim3(256), 0, ctx->stream, ns, pp.src_list, pp.q0,
                        pp.q1, pp.area, ctx->red.as<double>());
     f.G = ctx->red.as<double>();
   }
-  launch_finalize(kind, f, ctx->stream);
+  if (ops) {
+    if (ops_launch_finalize(kind, pp.m, f, ctx->stream) != LPM_OK)
+      return fail(ctx, LPM_ERR_ARG, "no epilogue for this operator and field count");
+  } else {
+    launch_finalize(kind, f, ctx->stream);
+  }
   LPM_HIP(ctx, hipGetLastError());
   return LPM_OK;
 }

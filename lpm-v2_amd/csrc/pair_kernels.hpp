@@ -27,16 +27,19 @@
 
 namespace lpm {
 
-// Source records: 8 words per source (FP32) or 8 doubles (FP64).
+// Source records: P::NW words per source (FP32) or doubles (FP64); 8 unless
+// a policy needs more.
 //   sphere BS   : x y z e w2 idx . .     e = R^2-|x|^2, w2 = -zeta A/(2 pi R)
 //   plane  BS   : x y w idx . . . .      w = omega A / (2 pi)
 //   PSE         : x y fA A idx . . .     coordinates scaled (see PsePolicy)
 //   sphere green: x y z e gr ga idx .    g = -zeta A ln2/(4 pi) (FP32) or -zeta A/(4 pi) (FP64)
 //   plane  green: x y g idx . . . .      g = omega A ln2/(4 pi) (FP32) or omega A/(4 pi) (FP64)
+//   PSE derivative, interpolation, SWE: see PseDeriv, PseInterp, SwePlane
 // Target records: 4 words per target: sphere {x y z e}, plane {x y . .},
 // PSE {x y f .}.
 constexpr int SREC = 8;
 constexpr int TREC = 4;
+constexpr int SREC_MAX = 12;
 
 template <typename T>
 struct Vec4 {
@@ -65,6 +68,7 @@ template <typename T, bool USE_E>
 struct SphereBS {
   static constexpr int NOUT = 3;
   static constexpr int IDX = 5;
+  static constexpr int NW = SREC;
   // acc += s * (x_i - y_j),  s = w2 / (|x_i - y_j|^2 + e_i + e_j); with
   // USE_E == false the points are on the sphere and e_i = e_j = 0.
   __device__ __forceinline__ static void arg(const Vec4<T>& t, const T* s, W4<T>& w) {
@@ -97,6 +101,7 @@ template <typename T>
 struct PlaneBS {
   static constexpr int NOUT = 2;
   static constexpr int IDX = 3;
+  static constexpr int NW = SREC;
   __device__ __forceinline__ static void arg(const Vec4<T>& t, const T* s, W4<T>& w) {
     w.a = t.a - s[0];
     w.b = t.b - s[1];
@@ -122,6 +127,7 @@ template <typename T, int ORDER>
 struct PsePolicy {
   static constexpr int NOUT = 1;
   static constexpr int IDX = 4;
+  static constexpr int NW = SREC;
   __device__ __forceinline__ static void arg(const Vec4<T>& t, const T* s, W4<T>& w) {
     T dx = t.a - s[0], dy = t.b - s[1];
     w.z = fma(dx, dx, dy * dy);
@@ -158,26 +164,35 @@ struct PsePolicy {
 // (the Taylor series of 4 asin^2(s / 2)), through n = 7; the caller uses it
 // when 7 eps <= 0.56 R, where the first dropped term is below 4e-9 relative.
 // Source record: {x', y', z', f A, A, idx, 1/(kR)^2, (kR)^2}; target {x', y', z', f}.
+// Scaled squared great-circle distance q = (k R theta)^2 from the squared
+// chord c2 = |x'_i - x'_j|^2 of directions scaled by k R; inv = 1/(k R)^2,
+// kr2 = (k R)^2.
+template <typename T, bool SERIES>
+__device__ __forceinline__ T sphere_q(T c2, T inv, T kr2) {
+  const T u = c2 * inv;
+  if constexpr (SERIES) {
+    T g = fma(u, (T)(1.0 / 84084.0), (T)(1.0 / 16632.0));
+    g = fma(u, g, (T)(1.0 / 3150.0));
+    g = fma(u, g, (T)(1.0 / 560.0));
+    g = fma(u, g, (T)(1.0 / 90.0));
+    g = fma(u, g, (T)(1.0 / 12.0));
+    g = fma(u, g, (T)1.0);
+    return c2 * g;
+  } else {
+    const T th = (T)2.0 * asin(fmin((T)0.5 * sqrt(u), (T)1.0));
+    return th * th * kr2;
+  }
+}
+
 template <typename T, bool SERIES>
 struct PseSphere {
   static constexpr int NOUT = 1;
   static constexpr int IDX = 5;
+  static constexpr int NW = SREC;
   __device__ __forceinline__ static void arg(const Vec4<T>& t, const T* s, W4<T>& w) {
     const T dx = t.a - s[0], dy = t.b - s[1], dz = t.c - s[2];
     const T c2 = fma(dx, dx, fma(dy, dy, dz * dz));
-    const T u = c2 * s[6];
-    if constexpr (SERIES) {
-      T g = fma(u, (T)(1.0 / 84084.0), (T)(1.0 / 16632.0));
-      g = fma(u, g, (T)(1.0 / 3150.0));
-      g = fma(u, g, (T)(1.0 / 560.0));
-      g = fma(u, g, (T)(1.0 / 90.0));
-      g = fma(u, g, (T)(1.0 / 12.0));
-      g = fma(u, g, (T)1.0);
-      w.z = c2 * g;
-    } else {
-      const T th = (T)2.0 * asin(fmin((T)0.5 * sqrt(u), (T)1.0));
-      w.z = th * th * s[7];
-    }
+    w.z = sphere_q<T, SERIES>(c2, s[6], s[7]);
     w.a = fma(-t.d, s[4], s[3]);  // f_j A_j - f_i A_j
   }
   __device__ __forceinline__ static T trans(const W4<T>& w) { return PsePolicy<T, 8>::trans(w); }
@@ -190,6 +205,7 @@ template <typename T, bool USE_E>
 struct SphereGreen {
   static constexpr int NOUT = 2;
   static constexpr int IDX = 6;
+  static constexpr int NW = SREC;
   __device__ __forceinline__ static void arg(const Vec4<T>& t, const T* s, W4<T>& w) {
     T dx = t.a - s[0], dy = t.b - s[1], dz = t.c - s[2];
     T r2;
@@ -216,6 +232,7 @@ template <typename T>
 struct PlaneGreen {
   static constexpr int NOUT = 1;
   static constexpr int IDX = 3;
+  static constexpr int NW = SREC;
   __device__ __forceinline__ static void arg(const Vec4<T>& t, const T* s, W4<T>& w) {
     T dx = t.a - s[0], dy = t.b - s[1];
     w.z = fma(dx, dx, dy * dy);
@@ -231,10 +248,147 @@ struct PlaneGreen {
   }
 };
 
+// Polynomial parts of the 8th-order PSE kernels in the scaled variable q
+// (q = rho^2 in FP64, log2(e) rho^2 in FP32, where exp(-rho^2) = exp2(-q)):
+//   derivative kernel (PSEDirectSum.f90:629-634): -20 + 20 r^2 - 5 r^4 + r^6/3
+//   delta kernel      (PSEDirectSum.f90:611-616):   4 -  6 r^2 + 2 r^4 - r^6/6
+//   Laplacian kernel  (PSEDirectSum.f90:622-627):  40 - 40 r^2 + 10 r^4 - 2 r^6/3
+template <typename T>
+__device__ __forceinline__ T cubic_in_q(T q, double c0, double c1, double c2, double c3) {
+  constexpr double L = sizeof(T) == 4 ? 1.4426950408889634 : 1.0;
+  return fma(fma(fma((T)(c3 / (L * L * L)), q, (T)(c2 / (L * L))), q, (T)(c1 / L)), q, (T)c0);
+}
+
+template <typename T>
+__device__ __forceinline__ T exp_neg_q(T q) {
+  if constexpr (sizeof(T) == 4)
+    return fast_exp2(-q);
+  else
+    return exp(-q);
+}
+
+// PSE first-derivative sums (PSEDirectSum.f90:180-418, 537-579).  For M source
+// fields c_m and the D coordinates of the separation X = x_i - x_j:
+//   a[k]             = sum_j X_k K1'(q) A_j           (k < D)
+//   a[D + m D + k]   = sum_j X_k K1'(q) c_m,j A_j
+// (K1' the unnormalised kernel, coordinates scaled by sigma).  Every
+// reference operator is an FP64 epilogue of these sums, since
+// sum (c_j + c_i) X K1 A = a[D + m D + k] + c_i a[k].
+// Source record: plane {x', y', A, c_0 .. c_{M-1}, idx};
+// sphere {x', y', z', A, c_0 .., idx, 1/(kR)^2, (kR)^2} with x' the unit
+// direction times k R (the separation is taken as R (d_i - d_j)).
+template <typename T, bool SPH, int M, bool SERIES>
+struct PseDeriv {
+  static constexpr int D = SPH ? 3 : 2;
+  static constexpr int NOUT = D * (M + 1);
+  static constexpr int IA = D;
+  static constexpr int IC = D + 1;
+  static constexpr int IDX = D + 1 + M;
+  static constexpr int IINV = IDX + 1;
+  static constexpr int IKR2 = IDX + 2;
+  static constexpr int NW = (SPH ? IKR2 + 1 : IDX + 1) <= SREC ? SREC : SREC_MAX;
+  static_assert((SPH ? IKR2 + 1 : IDX + 1) <= SREC_MAX, "record too long");
+  __device__ __forceinline__ static void arg(const Vec4<T>& t, const T* s, W4<T>& w) {
+    w.a = t.a - s[0];
+    w.b = t.b - s[1];
+    if constexpr (SPH) {
+      w.c = t.c - s[2];
+      const T c2 = fma(w.a, w.a, fma(w.b, w.b, w.c * w.c));
+      w.z = sphere_q<T, SERIES>(c2, s[IINV], s[IKR2]);
+    } else {
+      w.z = fma(w.a, w.a, w.b * w.b);
+    }
+  }
+  __device__ __forceinline__ static T trans(const W4<T>& w) { return exp_neg_q(w.z); }
+  __device__ __forceinline__ static void acc(const T* s, const W4<T>& w, T e, T a[NOUT]) {
+    const T g = cubic_in_q(w.z, -20.0, 20.0, -5.0, 1.0 / 3.0) * e * s[IA];
+    T X[D];
+    X[0] = w.a * g;
+    X[1] = w.b * g;
+    if constexpr (SPH) X[2] = w.c * g;
+#pragma unroll
+    for (int k = 0; k < D; ++k) {
+      a[k] += X[k];
+#pragma unroll
+      for (int m = 0; m < M; ++m) a[D + m * D + k] = fma(s[IC + m], X[k], a[D + m * D + k]);
+    }
+  }
+};
+
+// PSE interpolation (PSEDirectSum.f90:121-168): sum_j f_j A_j Kd'(q).
+// Source record: plane {x', y', f A, idx}; sphere {x', y', z', f A, idx,
+// 1/(kR)^2, (kR)^2}.
+template <typename T, bool SPH, bool SERIES>
+struct PseInterp {
+  static constexpr int D = SPH ? 3 : 2;
+  static constexpr int NOUT = 1;
+  static constexpr int IFA = D;
+  static constexpr int IDX = D + 1;
+  static constexpr int NW = SREC;
+  __device__ __forceinline__ static void arg(const Vec4<T>& t, const T* s, W4<T>& w) {
+    const T dx = t.a - s[0], dy = t.b - s[1];
+    if constexpr (SPH) {
+      const T dz = t.c - s[2];
+      w.z = sphere_q<T, SERIES>(fma(dx, dx, fma(dy, dy, dz * dz)), s[IDX + 1], s[IDX + 2]);
+    } else {
+      w.z = fma(dx, dx, dy * dy);
+    }
+  }
+  __device__ __forceinline__ static T trans(const W4<T>& w) { return exp_neg_q(w.z); }
+  __device__ __forceinline__ static void acc(const T* s, const W4<T>& w, T e, T a[NOUT]) {
+    a[0] = fma(cubic_in_q(w.z, 4.0, -6.0, 2.0, -1.0 / 6.0) * e, s[IFA], a[0]);
+  }
+};
+
+// Planar shallow-water right-hand side (SWEPlaneSolver.f90:457-538), one pass
+// over the pairs for seven sums: velocity (u, v), velocity gradient
+// (ux, uy, vx, vy) of the point vortices and sources, and the K8 Laplacian of
+// the surface height s = h + bottom.
+// Source record {x, y, a, b, s A, A, idx, c}: a = zeta A / (2 pi),
+// b = delta A / (2 pi), c = log2(e) / eps^2 (FP32) or 1 / eps^2 (FP64).
+// Target record {x, y, s, .}.  With f = 1 / r^2:
+//   u  += -Y a f + X b f             v  += X a f + Y b f
+//   ux += b f - 2 X (X b - Y a) f^2  uy += -a f - 2 Y (X b - Y a) f^2
+//   vx += a f - 2 X (Y b + X a) f^2  vy += b f - 2 Y (Y b + X a) f^2
+//   lap += K8'(r^2 c) (s_j - s_i) A_j
+template <typename T>
+struct SwePlane {
+  static constexpr int NOUT = 7;
+  static constexpr int IDX = 6;
+  static constexpr int NW = SREC;
+  __device__ __forceinline__ static void arg(const Vec4<T>& t, const T* s, W4<T>& w) {
+    w.a = t.a - s[0];
+    w.b = t.b - s[1];
+    w.z = fma(w.a, w.a, w.b * w.b);
+    w.c = fma(-t.c, s[5], s[4]);  // (s_j - s_i) A_j
+  }
+  __device__ __forceinline__ static T trans(const W4<T>& w) {
+    if constexpr (sizeof(T) == 4)
+      return fast_rcp(w.z);
+    else
+      return (T)1 / w.z;
+  }
+  __device__ __forceinline__ static void acc(const T* s, const W4<T>& w, T f, T a[NOUT]) {
+    const T X = w.a, Y = w.b;
+    const T rot = s[2] * f, pot = s[3] * f;
+    a[0] = fma(X, pot, fma(-Y, rot, a[0]));
+    a[1] = fma(Y, pot, fma(X, rot, a[1]));
+    const T hh = (f + f) * f;
+    const T p1 = fma(X, s[3], -Y * s[2]) * hh;
+    const T p2 = fma(Y, s[3], X * s[2]) * hh;
+    a[2] = fma(-X, p1, a[2] + pot);
+    a[3] = fma(-Y, p1, a[3] - rot);
+    a[4] = fma(-X, p2, a[4] + rot);
+    a[5] = fma(-Y, p2, a[5] + pot);
+    const T q = w.z * s[7];
+    a[6] = fma(cubic_in_q(q, 40.0, -40.0, 10.0, -2.0 / 3.0) * exp_neg_q(q), w.c, a[6]);
+  }
+};
+
 // ----------------------------------------------------------- the kernel
 struct PairArgs {
   const void* tgt;   // nt target records (TREC words each)
-  const void* src;   // ns source records (SREC words each)
+  const void* src;   // ns source records (P::NW words each)
   double* partial;   // [nsplit][NOUT][nt]
   int nt;            // number of targets
   int tgt_idx0;      // particle index of target 0 (self-pair exclusion)
@@ -296,8 +450,8 @@ __device__ __forceinline__ void pair_tile(const Vec4<T>* __restrict__ tg, int nt
 
   for (int jt = j_begin; jt < j_end; jt += kt) {
     const int jt_end = min(j_end, jt + kt);
-    const int idx_lo = rec_index(src + (size_t)jt * SREC, P::IDX);
-    const int idx_hi = rec_index(src + (size_t)(jt_end - 1) * SREC, P::IDX);
+    const int idx_lo = rec_index(src + (size_t)jt * P::NW, P::IDX);
+    const int idx_hi = rec_index(src + (size_t)(jt_end - 1) * P::NW, P::IDX);
     const bool checked = (idx_lo <= wt_hi) && (idx_hi >= wt_lo);
     T p[R][NO];
 #pragma unroll
@@ -308,10 +462,10 @@ __device__ __forceinline__ void pair_tile(const Vec4<T>* __restrict__ tg, int nt
     if (!checked) {
 #pragma unroll 2
       for (int j = jt; j < jt_end; ++j) {
-        const T* s = src + (size_t)j * SREC;
-        T sv[SREC];
+        const T* s = src + (size_t)j * P::NW;
+        T sv[P::NW];
 #pragma unroll
-        for (int q = 0; q < SREC; ++q) sv[q] = s[q];
+        for (int q = 0; q < P::NW; ++q) sv[q] = s[q];
         W4<T> w[R];
         T f[R];
 #pragma unroll
@@ -323,10 +477,10 @@ __device__ __forceinline__ void pair_tile(const Vec4<T>* __restrict__ tg, int nt
       }
     } else {
       for (int j = jt; j < jt_end; ++j) {
-        const T* s = src + (size_t)j * SREC;
-        T sv[SREC];
+        const T* s = src + (size_t)j * P::NW;
+        T sv[P::NW];
 #pragma unroll
-        for (int q = 0; q < SREC; ++q) sv[q] = s[q];
+        for (int q = 0; q < P::NW; ++q) sv[q] = s[q];
         const int sidx = rec_index(sv, P::IDX);
 #pragma unroll
         for (int r = 0; r < R; ++r) {

@@ -79,6 +79,118 @@ module lpm_direct_f
       real(c_double), value :: eps, radius
     end function
 
+    ! PSE operators and the planar SWE right-hand side (PSEDirectSum.f90:121-418, 537-579;
+    ! SWEPlaneSolver.f90:457-538)
+    integer(c_int) function lpm_pse_sphere_laplacian_at_points(ctx, lap, px, py, pz, pf, nt, x, y, z, f, &
+        area, active, n, eps, radius, precision) bind(C, name="lpm_pse_sphere_laplacian_at_points")
+      import :: c_int, c_ptr, c_double, c_int32_t
+      type(c_ptr), value :: ctx
+      real(c_double) :: lap(*)
+      real(c_double), intent(in) :: px(*), py(*), pz(*), pf(*), x(*), y(*), z(*), f(*), area(*)
+      integer(c_int32_t), intent(in) :: active(*)
+      integer(c_int), value :: nt, n, precision
+      real(c_double), value :: eps, radius
+    end function
+
+    integer(c_int) function lpm_pse_plane_gradient(ctx, gx, gy, x, y, f, area, active, n, i0, i1, eps, &
+        precision) bind(C, name="lpm_pse_plane_gradient")
+      import :: c_int, c_ptr, c_double, c_int32_t
+      type(c_ptr), value :: ctx
+      real(c_double) :: gx(*), gy(*)
+      real(c_double), intent(in) :: x(*), y(*), f(*), area(*)
+      integer(c_int32_t), intent(in) :: active(*)
+      integer(c_int), value :: n, i0, i1, precision
+      real(c_double), value :: eps
+    end function
+
+    integer(c_int) function lpm_pse_sphere_gradient(ctx, gx, gy, gz, x, y, z, f, area, active, n, i0, i1, &
+        eps, radius, precision) bind(C, name="lpm_pse_sphere_gradient")
+      import :: c_int, c_ptr, c_double, c_int32_t
+      type(c_ptr), value :: ctx
+      real(c_double) :: gx(*), gy(*), gz(*)
+      real(c_double), intent(in) :: x(*), y(*), z(*), f(*), area(*)
+      integer(c_int32_t), intent(in) :: active(*)
+      integer(c_int), value :: n, i0, i1, precision
+      real(c_double), value :: eps, radius
+    end function
+
+    integer(c_int) function lpm_pse_plane_second_partials(ctx, dxx, dxy, dyy, x, y, gx, gy, area, active, &
+        n, i0, i1, eps, precision) bind(C, name="lpm_pse_plane_second_partials")
+      import :: c_int, c_ptr, c_double, c_int32_t
+      type(c_ptr), value :: ctx
+      real(c_double) :: dxx(*), dxy(*), dyy(*)
+      real(c_double), intent(in) :: x(*), y(*), gx(*), gy(*), area(*)
+      integer(c_int32_t), intent(in) :: active(*)
+      integer(c_int), value :: n, i0, i1, precision
+      real(c_double), value :: eps
+    end function
+
+    integer(c_int) function lpm_pse_plane_double_dot(ctx, ddot, x, y, u, v, area, active, n, i0, i1, eps, &
+        precision) bind(C, name="lpm_pse_plane_double_dot")
+      import :: c_int, c_ptr, c_double, c_int32_t
+      type(c_ptr), value :: ctx
+      real(c_double) :: ddot(*)
+      real(c_double), intent(in) :: x(*), y(*), u(*), v(*), area(*)
+      integer(c_int32_t), intent(in) :: active(*)
+      integer(c_int), value :: n, i0, i1, precision
+      real(c_double), value :: eps
+    end function
+
+    integer(c_int) function lpm_pse_sphere_double_dot(ctx, ddot, x, y, z, u, v, w, area, active, n, i0, i1, &
+        eps, radius, precision) bind(C, name="lpm_pse_sphere_double_dot")
+      import :: c_int, c_ptr, c_double, c_int32_t
+      type(c_ptr), value :: ctx
+      real(c_double) :: ddot(*)
+      real(c_double), intent(in) :: x(*), y(*), z(*), u(*), v(*), w(*), area(*)
+      integer(c_int32_t), intent(in) :: active(*)
+      integer(c_int), value :: n, i0, i1, precision
+      real(c_double), value :: eps, radius
+    end function
+
+    integer(c_int) function lpm_pse_sphere_divergence(ctx, div, x, y, z, u, v, w, area, active, n, i0, i1, &
+        eps, radius, precision) bind(C, name="lpm_pse_sphere_divergence")
+      import :: c_int, c_ptr, c_double, c_int32_t
+      type(c_ptr), value :: ctx
+      real(c_double) :: div(*)
+      real(c_double), intent(in) :: x(*), y(*), z(*), u(*), v(*), w(*), area(*)
+      integer(c_int32_t), intent(in) :: active(*)
+      integer(c_int), value :: n, i0, i1, precision
+      real(c_double), value :: eps, radius
+    end function
+
+    integer(c_int) function lpm_pse_plane_interpolate(ctx, out, px, py, nt, x, y, f, area, active, n, eps, &
+        precision) bind(C, name="lpm_pse_plane_interpolate")
+      import :: c_int, c_ptr, c_double, c_int32_t
+      type(c_ptr), value :: ctx
+      real(c_double) :: out(*)
+      real(c_double), intent(in) :: px(*), py(*), x(*), y(*), f(*), area(*)
+      integer(c_int32_t), intent(in) :: active(*)
+      integer(c_int), value :: nt, n, precision
+      real(c_double), value :: eps
+    end function
+
+    integer(c_int) function lpm_pse_sphere_interpolate(ctx, out, px, py, pz, nt, x, y, z, f, area, active, &
+        n, eps, radius, precision) bind(C, name="lpm_pse_sphere_interpolate")
+      import :: c_int, c_ptr, c_double, c_int32_t
+      type(c_ptr), value :: ctx
+      real(c_double) :: out(*)
+      real(c_double), intent(in) :: px(*), py(*), pz(*), x(*), y(*), z(*), f(*), area(*)
+      integer(c_int32_t), intent(in) :: active(*)
+      integer(c_int), value :: nt, n, precision
+      real(c_double), value :: eps, radius
+    end function
+
+    integer(c_int) function lpm_swe_plane_rhs(ctx, u, v, ddot, lapsurf, x, y, vort, div, h, bottom, area, &
+        active, n, i0, i1, eps, precision) bind(C, name="lpm_swe_plane_rhs")
+      import :: c_int, c_ptr, c_double, c_int32_t
+      type(c_ptr), value :: ctx
+      real(c_double) :: u(*), v(*), ddot(*), lapsurf(*)
+      real(c_double), intent(in) :: x(*), y(*), vort(*), div(*), h(*), bottom(*), area(*)
+      integer(c_int32_t), intent(in) :: active(*)
+      integer(c_int), value :: n, i0, i1, precision
+      real(c_double), value :: eps
+    end function
+
     integer(c_int) function lpm_bve_sphere_stream(ctx, psi_rel, psi_abs, x, y, z, relvort, absvort, area, &
         radius, active, n, i0, i1, precision) bind(C, name="lpm_bve_sphere_stream")
       import :: c_int, c_ptr, c_double, c_int32_t

@@ -26,8 +26,11 @@ PREC_FP64 = 0
 PREC_FP32_KAHAN = 1
 PRECISIONS = {"fp64": PREC_FP64, "fp32": PREC_FP32_KAHAN, "fp32k": PREC_FP32_KAHAN}
 
-K_SPHERE_BS, K_PLANE_BS, K_PSE8, K_PSE2, K_SPHERE_GREEN, K_PLANE_GREEN, K_FAR_FIELD, K_PSE_SPHERE8 = range(8)
-KIND_NAMES = ["sphere_bs", "plane_bs", "pse8", "pse2", "sphere_green", "plane_green"]
+(K_SPHERE_BS, K_PLANE_BS, K_PSE8, K_PSE2, K_SPHERE_GREEN, K_PLANE_GREEN, K_FAR_FIELD, K_PSE_SPHERE8,
+ K_PSE_DERIV_PLANE, K_PSE_DERIV_SPHERE, K_PSE_INTERP_PLANE, K_PSE_INTERP_SPHERE, K_SWE_PLANE) = range(13)
+K_COUNT = 13
+KIND_NAMES = ["sphere_bs", "plane_bs", "pse8", "pse2", "sphere_green", "plane_green", "far_field", "pse_sphere8",
+              "pse_deriv_plane", "pse_deriv_sphere", "pse_interp_plane", "pse_interp_sphere", "swe_plane"]
 
 SOLVER_STREAM_EACH_STEP = 1
 SOLVER_USE_GRAPH = 2
@@ -58,6 +61,18 @@ _SIGS = {
     "lpm_plane_velocity": ([_vp, _pd, _pd, _pd, _pd, _pd, _pd, _pi, _i, _i, _i, _i], _i),
     "lpm_pse_plane_laplacian": ([_vp, _pd, _pd, _pd, _pd, _pd, _pi, _i, _i, _i, _d, _i, _i], _i),
     "lpm_pse_sphere_laplacian": ([_vp, _pd, _pd, _pd, _pd, _pd, _pd, _pi, _i, _i, _i, _d, _d, _i], _i),
+    "lpm_pse_sphere_laplacian_at_points": ([_vp, _pd, _pd, _pd, _pd, _pd, _i, _pd, _pd, _pd, _pd, _pd, _pi, _i, _d,
+                                            _d, _i], _i),
+    "lpm_pse_plane_gradient": ([_vp, _pd, _pd, _pd, _pd, _pd, _pd, _pi, _i, _i, _i, _d, _i], _i),
+    "lpm_pse_sphere_gradient": ([_vp, _pd, _pd, _pd, _pd, _pd, _pd, _pd, _pd, _pi, _i, _i, _i, _d, _d, _i], _i),
+    "lpm_pse_plane_second_partials": ([_vp, _pd, _pd, _pd, _pd, _pd, _pd, _pd, _pd, _pi, _i, _i, _i, _d, _i], _i),
+    "lpm_pse_plane_double_dot": ([_vp, _pd, _pd, _pd, _pd, _pd, _pd, _pi, _i, _i, _i, _d, _i], _i),
+    "lpm_pse_sphere_double_dot": ([_vp, _pd, _pd, _pd, _pd, _pd, _pd, _pd, _pd, _pi, _i, _i, _i, _d, _d, _i], _i),
+    "lpm_pse_sphere_divergence": ([_vp, _pd, _pd, _pd, _pd, _pd, _pd, _pd, _pd, _pi, _i, _i, _i, _d, _d, _i], _i),
+    "lpm_pse_plane_interpolate": ([_vp, _pd, _pd, _pd, _i, _pd, _pd, _pd, _pd, _pi, _i, _d, _i], _i),
+    "lpm_pse_sphere_interpolate": ([_vp, _pd, _pd, _pd, _pd, _i, _pd, _pd, _pd, _pd, _pd, _pi, _i, _d, _d, _i], _i),
+    "lpm_swe_plane_rhs": ([_vp, _pd, _pd, _pd, _pd, _pd, _pd, _pd, _pd, _pd, _pd, _pd, _pi, _i, _i, _i, _d, _i],
+                          _i),
     "lpm_bve_sphere_stream": ([_vp, _pd, _pd, _pd, _pd, _pd, _pd, _pd, _pd, _d, _pi, _i, _i, _i, _i], _i),
     "lpm_plane_stream": ([_vp, _pd, _pd, _pd, _pd, _pd, _pi, _i, _i, _i, _i], _i),
     "lpm_pse_eps": ([_d, _d, _pd], _i),

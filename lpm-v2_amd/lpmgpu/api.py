@@ -105,6 +105,109 @@ class Context:
                                                       pi(act), n, i0, i1, eps, radius, prec_code(precision)))
         return lap
 
+    def pse_sphere_laplacian_at_points(self, x, y, z, f, area, active, eps, px, py, pz, pf, radius=1.0,
+                                       precision="fp32"):
+        x, y, z, f, a, px, py, pz, pf = map(f64, (x, y, z, f, area, px, py, pz, pf))
+        act = i32(active)
+        out = np.zeros(px.shape[0])
+        self.check(self._lib.lpm_pse_sphere_laplacian_at_points(
+            self.handle, pd(out), pd(px), pd(py), pd(pz), pd(pf), px.shape[0], pd(x), pd(y), pd(z), pd(f), pd(a),
+            pi(act), x.shape[0], eps, radius, prec_code(precision)))
+        return out
+
+    def pse_plane_gradient(self, x, y, f, area, active, eps, i0=0, i1=None, precision="fp32"):
+        x, y, f, a = map(f64, (x, y, f, area))
+        act = i32(active)
+        n = x.shape[0]
+        i1 = n if i1 is None else i1
+        gx, gy = np.zeros(n), np.zeros(n)
+        self.check(self._lib.lpm_pse_plane_gradient(self.handle, pd(gx), pd(gy), pd(x), pd(y), pd(f), pd(a),
+                                                    pi(act), n, i0, i1, eps, prec_code(precision)))
+        return gx, gy
+
+    def pse_sphere_gradient(self, x, y, z, f, area, active, eps, radius=1.0, i0=0, i1=None, precision="fp32"):
+        x, y, z, f, a = map(f64, (x, y, z, f, area))
+        act = i32(active)
+        n = x.shape[0]
+        i1 = n if i1 is None else i1
+        g = [np.zeros(n) for _ in range(3)]
+        self.check(self._lib.lpm_pse_sphere_gradient(self.handle, *map(pd, g), pd(x), pd(y), pd(z), pd(f), pd(a),
+                                                     pi(act), n, i0, i1, eps, radius, prec_code(precision)))
+        return tuple(g)
+
+    def pse_plane_second_partials(self, x, y, gx, gy, area, active, eps, i0=0, i1=None, precision="fp32"):
+        x, y, gx, gy, a = map(f64, (x, y, gx, gy, area))
+        act = i32(active)
+        n = x.shape[0]
+        i1 = n if i1 is None else i1
+        d = [np.zeros(n) for _ in range(3)]
+        self.check(self._lib.lpm_pse_plane_second_partials(self.handle, *map(pd, d), pd(x), pd(y), pd(gx), pd(gy),
+                                                           pd(a), pi(act), n, i0, i1, eps, prec_code(precision)))
+        return tuple(d)
+
+    def pse_plane_double_dot(self, x, y, u, v, area, active, eps, i0=0, i1=None, precision="fp32"):
+        x, y, u, v, a = map(f64, (x, y, u, v, area))
+        act = i32(active)
+        n = x.shape[0]
+        i1 = n if i1 is None else i1
+        dd = np.zeros(n)
+        self.check(self._lib.lpm_pse_plane_double_dot(self.handle, pd(dd), pd(x), pd(y), pd(u), pd(v), pd(a),
+                                                      pi(act), n, i0, i1, eps, prec_code(precision)))
+        return dd
+
+    def pse_sphere_double_dot(self, x, y, z, u, v, w, area, active, eps, radius=1.0, i0=0, i1=None,
+                              precision="fp32"):
+        x, y, z, u, v, w, a = map(f64, (x, y, z, u, v, w, area))
+        act = i32(active)
+        n = x.shape[0]
+        i1 = n if i1 is None else i1
+        dd = np.zeros(n)
+        self.check(self._lib.lpm_pse_sphere_double_dot(self.handle, pd(dd), pd(x), pd(y), pd(z), pd(u), pd(v),
+                                                       pd(w), pd(a), pi(act), n, i0, i1, eps, radius,
+                                                       prec_code(precision)))
+        return dd
+
+    def pse_sphere_divergence(self, x, y, z, u, v, w, area, active, eps, radius=1.0, i0=0, i1=None,
+                              precision="fp32"):
+        x, y, z, u, v, w, a = map(f64, (x, y, z, u, v, w, area))
+        act = i32(active)
+        n = x.shape[0]
+        i1 = n if i1 is None else i1
+        dv = np.zeros(n)
+        self.check(self._lib.lpm_pse_sphere_divergence(self.handle, pd(dv), pd(x), pd(y), pd(z), pd(u), pd(v),
+                                                       pd(w), pd(a), pi(act), n, i0, i1, eps, radius,
+                                                       prec_code(precision)))
+        return dv
+
+    def pse_plane_interpolate(self, x, y, f, area, active, eps, px, py, precision="fp32"):
+        x, y, f, a, px, py = map(f64, (x, y, f, area, px, py))
+        act = i32(active)
+        out = np.zeros(px.shape[0])
+        self.check(self._lib.lpm_pse_plane_interpolate(self.handle, pd(out), pd(px), pd(py), px.shape[0], pd(x),
+                                                       pd(y), pd(f), pd(a), pi(act), x.shape[0], eps,
+                                                       prec_code(precision)))
+        return out
+
+    def pse_sphere_interpolate(self, x, y, z, f, area, active, eps, px, py, pz, radius=1.0, precision="fp32"):
+        x, y, z, f, a, px, py, pz = map(f64, (x, y, z, f, area, px, py, pz))
+        act = i32(active)
+        out = np.zeros(px.shape[0])
+        self.check(self._lib.lpm_pse_sphere_interpolate(self.handle, pd(out), pd(px), pd(py), pd(pz), px.shape[0],
+                                                        pd(x), pd(y), pd(z), pd(f), pd(a), pi(act), x.shape[0], eps,
+                                                        radius, prec_code(precision)))
+        return out
+
+    def swe_plane_rhs(self, x, y, vort, div, h, bottom, area, active, eps, i0=0, i1=None, precision="fp32"):
+        """SWEPlaneRHSIntegrals: returns (u, v, double_dot, lap_surf); bottom = topoFn(x, y)."""
+        x, y, q, d, h, b, a = map(f64, (x, y, vort, div, h, bottom, area))
+        act = i32(active)
+        n = x.shape[0]
+        i1 = n if i1 is None else i1
+        out = [np.zeros(n) for _ in range(4)]
+        self.check(self._lib.lpm_swe_plane_rhs(self.handle, *map(pd, out), pd(x), pd(y), pd(q), pd(d), pd(h), pd(b),
+                                               pd(a), pi(act), n, i0, i1, eps, prec_code(precision)))
+        return tuple(out)
+
     def bve_sphere_stream(self, x, y, z, relvort, absvort, area, active, radius=1.0, i0=0, i1=None,
                           precision="fp32"):
         x, y, z, qr, qa, a = map(f64, (x, y, z, relvort, absvort, area))
