@@ -167,7 +167,7 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=int(os.environ.get("OMP_NUM_THREADS", "16")))
     ap.add_argument("--graph", action="store_true", help="replay steps as hipGraphs (no per-kernel timing)")
     ap.add_argument("--workload", default="n655k",
-                    help="n655k (default, the headline line) or cfg0/cfg1/cfg2/cfg3 for the other workloads")
+                    help="n655k (default, the headline line) or cfg0/cfg1/cfg2/cfg3/pse_ops for the other workloads")
     ap.add_argument("--cfg-steps", type=int, default=0, help="override the step count of cfg1/cfg3")
     ap.add_argument("--ff-steps", type=int, default=100, help="learned far-field RK4 steps (0 = skip)")
     ap.add_argument("--ff-samples", type=int, default=10_000_000, help="far-field training samples")
@@ -199,7 +199,7 @@ def main():
 
         ctx = lpmgpu.Context(0)
         fn = {"cfg0": configs.run_cfg0, "cfg1": configs.run_cfg1, "cfg2": configs.run_cfg2,
-              "cfg3": configs.run_cfg3}[args.workload]
+              "cfg3": configs.run_cfg3, "pse_ops": configs.run_pse_ops}[args.workload]
         kw = {"steps": args.cfg_steps} if args.cfg_steps and args.workload in ("cfg1", "cfg3") else {}
         res = fn(ctx, log=log, **kw)
         res["metric"] = METRIC

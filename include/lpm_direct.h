@@ -257,6 +257,19 @@ int lpm_bve_solver_new(lpm_ctx *ctx, lpm_bve_solver **solver, int n, const doubl
                        const double *absvort, const double *area, const int32_t *active,
                        const double *u, const double *v, const double *w, double radius,
                        double rotation_rate, int precision, int flags);
+/* Remesh re-upload: replace the solver's particles with a new set of n
+ * (the output of LagrangianRemeshBVEWithVorticityFunction, SsrfpackRemesh.f90:388-,
+ * called every remeshInterval steps, BVESingleGaussianVortex.f90:218-246).
+ * Radius, rotation rate, precision and flags are kept; device buffers are
+ * reused and grown only when n exceeds their size; a captured step graph is
+ * rebuilt at the next step.  u, v, w may be NULL (all three): the velocity
+ * is then computed on the device, as SetVelocityOnMesh does after a remesh
+ * (SsrfpackRemesh.f90:552).  After a failure the solver reports
+ * LPM_ERR_STATE until a reload succeeds. */
+int lpm_bve_solver_set_particles(lpm_bve_solver *solver, int n, const double *x, const double *y,
+                                 const double *z, const double *relvort, const double *absvort,
+                                 const double *area, const int32_t *active, const double *u,
+                                 const double *v, const double *w);
 /* One classical RK4 step (timestepPrivate, SphereBVESolver.f90:219-353). */
 int lpm_bve_solver_timestep(lpm_bve_solver *solver, double dt);
 int lpm_bve_solver_steps(lpm_bve_solver *solver, double dt, int nsteps);
@@ -277,6 +290,11 @@ int lpm_plane_solver_new(lpm_ctx *ctx, lpm_plane_solver **solver, int n, const d
                          const double *y, const double *vort, const double *area,
                          const int32_t *active, const double *u, const double *v,
                          int precision, int flags);
+/* Remesh re-upload for the planar solver (BIVARRemesh.f90 rebuilds the
+ * particles on the host); same contract as lpm_bve_solver_set_particles. */
+int lpm_plane_solver_set_particles(lpm_plane_solver *solver, int n, const double *x, const double *y,
+                                   const double *vort, const double *area, const int32_t *active,
+                                   const double *u, const double *v);
 int lpm_plane_solver_timestep(lpm_plane_solver *solver, double dt);
 int lpm_plane_solver_steps(lpm_plane_solver *solver, double dt, int nsteps);
 int lpm_plane_solver_get_state(lpm_plane_solver *solver, double *x, double *y, double *u,
@@ -408,6 +426,16 @@ int lpm_bve_ff_steps(lpm_bve_ff *ff, double dt, int nsteps, int mode, int judge_
                      double tol, int *mode_out, double *last_error, int *judged);
 int lpm_bve_ff_get_state(lpm_bve_ff *ff, double *x, double *y, double *z, double *relvort,
                          double *u, double *v, double *w);
+/* Remesh re-upload: a new particle set of n (any size) with its areas and
+ * active flags (LagrangianRemeshBVEWithVorticityFunction, SsrfpackRemesh.f90:388-).
+ * The cells (fixed regions of the sphere), their frames, the trained network
+ * and the collected samples are kept; the cell moments follow the new
+ * particles at every evaluation.  The plain velocity of the new set is
+ * stored.  After a failure the object reports LPM_ERR_STATE until a reload
+ * succeeds. */
+int lpm_bve_ff_set_particles(lpm_bve_ff *ff, int n, const double *x, const double *y,
+                             const double *z, const double *relvort, const double *area,
+                             const int32_t *active);
 /* Replace positions (projected onto the sphere) and relative vorticity, and
  * store the plain velocity of the new state; cells, anchors, set-up radii, the
  * network and the collected samples are kept. */

@@ -1053,6 +1053,7 @@ struct lpm_bve_ff {
   double ys = 0.0;  // RMS of the training targets (0: not fixed yet)
   int online_n = 0, online_epochs = 1, online_batch = 4096, online_rounds = 0;
   double online_lr = 2e-4;
+  bool loaded = false;  // false after a failed particle reload
 };
 
 namespace {
@@ -1208,41 +1209,17 @@ int ff_error(lpm_bve_ff* f, const double* au, const double* av, const double* aw
   return LPM_OK;
 }
 
-}  // namespace
-
-extern "C" {
-
-int lpm_bve_ff_new(lpm_ctx* ctx, lpm_bve_ff** out, int n, const double* x, const double* y, const double* z,
-                   const double* relvort, const double* area, const int32_t* active, const int32_t* cell, int ncell,
-                   double radius, double rotation_rate, double near_radius, double band_radius, int flags) {
-  if (!ctx || !out) return LPM_ERR_ARG;
-  ctx->err.clear();
-  *out = nullptr;
-  if (n <= 0 || ncell <= 0 || !x || !y || !z || !relvort || !area || !active || !cell)
-    return fail(ctx, LPM_ERR_ARG, "lpm_bve_ff_new: bad arguments");
-  if (!(radius > 0.0) || !(near_radius >= 0.0) || !(band_radius >= near_radius))
-    return fail(ctx, LPM_ERR_ARG, "lpm_bve_ff_new: need radius > 0 and 0 <= near_radius <= band_radius");
-  const int order = (flags & LPM_FF_ORDER_MASK) == 0 ? 4 : (flags & LPM_FF_ORDER_MASK);
-  if (order < 2 || order > 4 || (flags & ~LPM_FF_ORDER_MASK))
-    return fail(ctx, LPM_ERR_ARG, "lpm_bve_ff_new: flags hold the expansion order 2, 3 or 4 (0: 4) only");
-  for (int i = 0; i < n; ++i)
-    if (cell[i] < 0 || cell[i] >= ncell) return fail(ctx, LPM_ERR_ARG, "lpm_bve_ff_new: cell id out of range");
-  LPM_HIP(ctx, hipSetDevice(ctx->device));
-  lpm_bve_ff* f = new lpm_bve_ff();
-  f->ctx = ctx;
+// Load a particle set (at creation, and after a remesh): everything sized by
+// the particle count -- active list, work-item bounds, near-list splits, the
+// radix-sort workspace and the particle arrays (grown, never shrunk) -- then
+// the plain velocity of the set (as BVESolver New) and its binning.  The
+// cells, their frames, the network and the collected samples are kept.
+int ff_load(lpm_bve_ff* f, int n, const double* x, const double* y, const double* z, const double* relvort,
+            const double* area, const int32_t* active) {
+  lpm_ctx* ctx = f->ctx;
+  const int ncell = f->ncell;
+  f->loaded = false;
   f->n = n;
-  f->ncell = ncell;
-  f->radius = radius;
-  f->omega = rotation_rate;
-  f->r_near = near_radius;
-  f->r_band = band_radius;
-  f->order = order;
-  // Cells are fixed regions of the sphere: each is the set of points nearest to
-  // its generator, the direction of the area-weighted centroid of the given
-  // cell's active sources at set-up (of all its particles if it has none).  Its
-  // frame anchor is the direction of its active source farthest from that
-  // centroid (lowest particle index on ties).  Particles are binned anew at
-  // every evaluation, so cells never stretch with the flow.
   std::vector<int> plain;
   std::vector<int32_t> act((size_t)n);
   for (int i = 0; i < n; ++i) {
@@ -1250,46 +1227,6 @@ int lpm_bve_ff_new(lpm_ctx* ctx, lpm_bve_ff** out, int n, const double* x, const
     if (active[i]) plain.push_back(i);
   }
   f->ns = (int)plain.size();
-  std::vector<lpm::Vec4<float>> gen((size_t)ncell);
-  std::vector<double> adir((size_t)ncell * 3, 0.0);
-  {
-    std::vector<double> cs((size_t)ncell * 4, 0.0), ca((size_t)ncell * 4, 0.0);
-    for (int i = 0; i < n; ++i) {
-      double* c = &ca[(size_t)cell[i] * 4];
-      c[0] += x[i];
-      c[1] += y[i];
-      c[2] += z[i];
-      c[3] += 1.0;
-      if (active[i]) {
-        double* s = &cs[(size_t)cell[i] * 4];
-        s[0] += area[i] * x[i];
-        s[1] += area[i] * y[i];
-        s[2] += area[i] * z[i];
-        s[3] += area[i];
-      }
-    }
-    std::vector<double> best((size_t)ncell, -1.0);
-    for (int i = 0; i < n; ++i)
-      if (active[i]) {
-        const double* c = &cs[(size_t)cell[i] * 4];
-        const double inv = c[3] > 0 ? 1.0 / c[3] : 0.0;
-        const double dx = x[i] - c[0] * inv, dy = y[i] - c[1] * inv, dz = z[i] - c[2] * inv;
-        const double d2 = dx * dx + dy * dy + dz * dz;
-        if (d2 > best[(size_t)cell[i]]) {
-          best[(size_t)cell[i]] = d2;
-          const double r = std::sqrt(x[i] * x[i] + y[i] * y[i] + z[i] * z[i]);
-          adir[(size_t)cell[i] * 3 + 0] = x[i] / r;
-          adir[(size_t)cell[i] * 3 + 1] = y[i] / r;
-          adir[(size_t)cell[i] * 3 + 2] = z[i] / r;
-        }
-      }
-    for (int c = 0; c < ncell; ++c) {
-      const double* s = cs[(size_t)c * 4 + 3] > 0 ? &cs[(size_t)c * 4] : &ca[(size_t)c * 4];
-      const double r = std::sqrt(s[0] * s[0] + s[1] * s[1] + s[2] * s[2]);
-      if (r > 0) gen[(size_t)c] = lpm::Vec4<float>{(float)(s[0] / r), (float)(s[1] / r), (float)(s[2] / r), 0.f};
-      else gen[(size_t)c] = lpm::Vec4<float>{0.f, 0.f, 0.f, 0.f};  // empty cell: never nearest
-    }
-  }
   // work-item upper bounds: per cell ceil(count / tile) <= count / tile + 1
   f->n_items = ncell + n / (64 * kR) + 1;
   f->n_mlp_items = ncell + n / kMlpTile + 1;
@@ -1332,8 +1269,6 @@ int lpm_bve_ff_new(lpm_ctx* ctx, lpm_bve_ff** out, int n, const double* x, const
       chk(ff_upload(ctx, f->iota, nullptr, ni)) && chk(ff_upload(ctx, f->cellof, nullptr, ni)) &&
       chk(ff_upload(ctx, f->active, act.data(), ni)) &&
       chk(ff_upload(ctx, f->sort_tmp, nullptr, std::max<size_t>(sort_bytes, 16))) &&
-      chk(ff_upload(ctx, f->gen, gen.data(), (size_t)ncell * sizeof(lpm::Vec4<float>))) &&
-      chk(ff_upload(ctx, f->adir, adir.data(), adir.size() * 8)) &&
       chk(ff_upload(ctx, f->item_count, nullptr, 2 * 4)) &&
       chk(ff_upload(ctx, f->list_plain, plain.data(), std::max<size_t>(plain.size(), 1) * 4)) &&
       chk(ff_upload(ctx, f->items, nullptr, (size_t)f->n_items * sizeof(int2))) &&
@@ -1361,11 +1296,10 @@ int lpm_bve_ff_new(lpm_ctx* ctx, lpm_bve_ff** out, int n, const double* x, const
     hipError_t e = hipStreamSynchronize(ctx->stream);
     if (e != hipSuccess) rc = lpm::hip_fail(ctx, e, "hipStreamSynchronize");
   }
-  if (rc == LPM_OK) chk(f->net.init(ctx, 13, 128, 3, 3, LPM_ACT_ISRU, 12345));
   if (rc == LPM_OK) {
     // start on the sphere, with the plain velocity stored (as BVESolver New)
-    hipLaunchKernelGGL(ff_project, dim3((n + 255) / 256), dim3(256), 0, ctx->stream, n, radius, f->x.as<double>(),
-                       f->y.as<double>(), f->z.as<double>());
+    hipLaunchKernelGGL(ff_project, dim3((n + 255) / 256), dim3(256), 0, ctx->stream, n, f->radius,
+                       f->x.as<double>(), f->y.as<double>(), f->z.as<double>());
     chk(ff_eval(f, 0, f->x.as<double>(), f->y.as<double>(), f->z.as<double>(), f->q.as<double>(), f->u.as<double>(),
                 f->v.as<double>(), f->w.as<double>()));
     // bin once so that the cell queries are defined before any evaluation
@@ -1375,6 +1309,98 @@ int lpm_bve_ff_new(lpm_ctx* ctx, lpm_bve_ff** out, int n, const double* x, const
     hipError_t e = hipStreamSynchronize(ctx->stream);
     if (e != hipSuccess) rc = lpm::hip_fail(ctx, e, "hipStreamSynchronize");
   }
+  f->loaded = rc == LPM_OK;
+  return rc;
+}
+
+int ff_usable(lpm_bve_ff* f) {
+  if (f->loaded) return LPM_OK;
+  return fail(f->ctx, LPM_ERR_STATE, "far-field particles were not loaded (a particle reload failed)");
+}
+
+}  // namespace
+
+extern "C" {
+
+int lpm_bve_ff_new(lpm_ctx* ctx, lpm_bve_ff** out, int n, const double* x, const double* y, const double* z,
+                   const double* relvort, const double* area, const int32_t* active, const int32_t* cell, int ncell,
+                   double radius, double rotation_rate, double near_radius, double band_radius, int flags) {
+  if (!ctx || !out) return LPM_ERR_ARG;
+  ctx->err.clear();
+  *out = nullptr;
+  if (n <= 0 || ncell <= 0 || !x || !y || !z || !relvort || !area || !active || !cell)
+    return fail(ctx, LPM_ERR_ARG, "lpm_bve_ff_new: bad arguments");
+  if (!(radius > 0.0) || !(near_radius >= 0.0) || !(band_radius >= near_radius))
+    return fail(ctx, LPM_ERR_ARG, "lpm_bve_ff_new: need radius > 0 and 0 <= near_radius <= band_radius");
+  const int order = (flags & LPM_FF_ORDER_MASK) == 0 ? 4 : (flags & LPM_FF_ORDER_MASK);
+  if (order < 2 || order > 4 || (flags & ~LPM_FF_ORDER_MASK))
+    return fail(ctx, LPM_ERR_ARG, "lpm_bve_ff_new: flags hold the expansion order 2, 3 or 4 (0: 4) only");
+  for (int i = 0; i < n; ++i)
+    if (cell[i] < 0 || cell[i] >= ncell) return fail(ctx, LPM_ERR_ARG, "lpm_bve_ff_new: cell id out of range");
+  LPM_HIP(ctx, hipSetDevice(ctx->device));
+  lpm_bve_ff* f = new lpm_bve_ff();
+  f->ctx = ctx;
+  f->n = n;
+  f->ncell = ncell;
+  f->radius = radius;
+  f->omega = rotation_rate;
+  f->r_near = near_radius;
+  f->r_band = band_radius;
+  f->order = order;
+  // Cells are fixed regions of the sphere: each is the set of points nearest to
+  // its generator, the direction of the area-weighted centroid of the given
+  // cell's active sources at set-up (of all its particles if it has none).  Its
+  // frame anchor is the direction of its active source farthest from that
+  // centroid (lowest particle index on ties).  Particles are binned anew at
+  // every evaluation, so cells never stretch with the flow.
+  std::vector<lpm::Vec4<float>> gen((size_t)ncell);
+  std::vector<double> adir((size_t)ncell * 3, 0.0);
+  {
+    std::vector<double> cs((size_t)ncell * 4, 0.0), ca((size_t)ncell * 4, 0.0);
+    for (int i = 0; i < n; ++i) {
+      double* c = &ca[(size_t)cell[i] * 4];
+      c[0] += x[i];
+      c[1] += y[i];
+      c[2] += z[i];
+      c[3] += 1.0;
+      if (active[i]) {
+        double* s = &cs[(size_t)cell[i] * 4];
+        s[0] += area[i] * x[i];
+        s[1] += area[i] * y[i];
+        s[2] += area[i] * z[i];
+        s[3] += area[i];
+      }
+    }
+    std::vector<double> best((size_t)ncell, -1.0);
+    for (int i = 0; i < n; ++i)
+      if (active[i]) {
+        const double* c = &cs[(size_t)cell[i] * 4];
+        const double inv = c[3] > 0 ? 1.0 / c[3] : 0.0;
+        const double dx = x[i] - c[0] * inv, dy = y[i] - c[1] * inv, dz = z[i] - c[2] * inv;
+        const double d2 = dx * dx + dy * dy + dz * dz;
+        if (d2 > best[(size_t)cell[i]]) {
+          best[(size_t)cell[i]] = d2;
+          const double r = std::sqrt(x[i] * x[i] + y[i] * y[i] + z[i] * z[i]);
+          adir[(size_t)cell[i] * 3 + 0] = x[i] / r;
+          adir[(size_t)cell[i] * 3 + 1] = y[i] / r;
+          adir[(size_t)cell[i] * 3 + 2] = z[i] / r;
+        }
+      }
+    for (int c = 0; c < ncell; ++c) {
+      const double* s = cs[(size_t)c * 4 + 3] > 0 ? &cs[(size_t)c * 4] : &ca[(size_t)c * 4];
+      const double r = std::sqrt(s[0] * s[0] + s[1] * s[1] + s[2] * s[2]);
+      if (r > 0) gen[(size_t)c] = lpm::Vec4<float>{(float)(s[0] / r), (float)(s[1] / r), (float)(s[2] / r), 0.f};
+      else gen[(size_t)c] = lpm::Vec4<float>{0.f, 0.f, 0.f, 0.f};  // empty cell: never nearest
+    }
+  }
+  int rc = LPM_OK;
+  auto chk = [&](int r) {
+    if (r && !rc) rc = r;
+    return rc == LPM_OK;
+  };
+  chk(ff_upload(ctx, f->gen, gen.data(), (size_t)ncell * sizeof(lpm::Vec4<float>))) &&
+      chk(ff_upload(ctx, f->adir, adir.data(), adir.size() * 8)) && chk(f->net.init(ctx, 13, 128, 3, 3, LPM_ACT_ISRU, 12345)) &&
+      chk(ff_load(f, n, x, y, z, relvort, area, active));
   if (rc != LPM_OK) {
     lpm_bve_ff_delete(f);
     return rc;
@@ -1387,6 +1413,7 @@ int lpm_bve_ff_velocity(lpm_bve_ff* f, int mode, double* u, double* v, double* w
   if (!f || !u || !v || !w) return LPM_ERR_ARG;
   lpm_ctx* ctx = f->ctx;
   ctx->err.clear();
+  if (int rc = ff_usable(f)) return rc;
   if (mode < 0 || mode > 2) return fail(ctx, LPM_ERR_ARG, "lpm_bve_ff_velocity: mode must be 0, 1 or 2");
   if (mode == 2 && !f->trained) return fail(ctx, LPM_ERR_STATE, "lpm_bve_ff_velocity: the network is not trained");
   LPM_HIP(ctx, hipSetDevice(ctx->device));
@@ -1405,6 +1432,7 @@ int lpm_bve_ff_collect(lpm_bve_ff* f, int n_samples, uint64_t seed, int64_t* tot
   if (!f || n_samples <= 0) return LPM_ERR_ARG;
   lpm_ctx* ctx = f->ctx;
   ctx->err.clear();
+  if (int rc = ff_usable(f)) return rc;
   LPM_HIP(ctx, hipSetDevice(ctx->device));
   // refresh geometry, moments and lists at the current state (expansion mode)
   if (int rc = ff_eval(f, 1, f->x.as<double>(), f->y.as<double>(), f->z.as<double>(), f->q.as<double>(),
@@ -1520,6 +1548,7 @@ int lpm_bve_ff_steps(lpm_bve_ff* f, double dt, int nsteps, int mode, int judge_e
   if (!f || nsteps < 0) return LPM_ERR_ARG;
   lpm_ctx* ctx = f->ctx;
   ctx->err.clear();
+  if (int rc = ff_usable(f)) return rc;
   if (mode < 0 || mode > 2) return fail(ctx, LPM_ERR_ARG, "lpm_bve_ff_steps: mode must be 0, 1 or 2");
   if (mode == 2 && !f->trained) return fail(ctx, LPM_ERR_STATE, "lpm_bve_ff_steps: the network is not trained");
   LPM_HIP(ctx, hipSetDevice(ctx->device));
@@ -1564,6 +1593,8 @@ int lpm_bve_ff_get_state(lpm_bve_ff* f, double* x, double* y, double* z, double*
                          double* w) {
   if (!f) return LPM_ERR_ARG;
   lpm_ctx* ctx = f->ctx;
+  ctx->err.clear();
+  if (int rc = ff_usable(f)) return rc;
   const DevBuf* src[7] = {&f->x, &f->y, &f->z, &f->q, &f->u, &f->v, &f->w};
   double* dst[7] = {x, y, z, relvort, u, v, w};
   for (int k = 0; k < 7; ++k)
@@ -1573,10 +1604,23 @@ int lpm_bve_ff_get_state(lpm_bve_ff* f, double* x, double* y, double* z, double*
   return LPM_OK;
 }
 
+int lpm_bve_ff_set_particles(lpm_bve_ff* f, int n, const double* x, const double* y, const double* z,
+                             const double* relvort, const double* area, const int32_t* active) {
+  if (!f) return LPM_ERR_ARG;
+  lpm_ctx* ctx = f->ctx;
+  ctx->err.clear();
+  if (n <= 0 || !x || !y || !z || !relvort || !area || !active)
+    return fail(ctx, LPM_ERR_ARG, "lpm_bve_ff_set_particles: n must be > 0 and the arrays non-null");
+  LPM_HIP(ctx, hipSetDevice(ctx->device));
+  if (int rc = ff_load(f, n, x, y, z, relvort, area, active)) return rc;
+  return lpm::collect_timing(ctx);
+}
+
 int lpm_bve_ff_set_state(lpm_bve_ff* f, const double* x, const double* y, const double* z, const double* relvort) {
   if (!f || !x || !y || !z || !relvort) return LPM_ERR_ARG;
   lpm_ctx* ctx = f->ctx;
   ctx->err.clear();
+  if (int rc = ff_usable(f)) return rc;
   LPM_HIP(ctx, hipSetDevice(ctx->device));
   const size_t nb = (size_t)f->n * sizeof(double);
   LPM_HIP(ctx, hipMemcpyAsync(f->x.p, x, nb, hipMemcpyHostToDevice, ctx->stream));

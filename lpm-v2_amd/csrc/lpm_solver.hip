@@ -171,6 +171,7 @@ struct lpm_bve_solver {
   hipGraphExec_t exec = nullptr;
   double graph_dt = 0.0;
   bool has_abs = false;
+  bool loaded = false;  // false after a failed particle reload
 };
 
 namespace {
@@ -299,34 +300,37 @@ int bve_step(lpm_bve_solver* s, double dt) {
   return LPM_OK;
 }
 
-}  // namespace
+// A captured step holds the sizes and buffer addresses of its capture.
+template <class Solver>
+void drop_graph(Solver* s) {
+  if (s->exec) (void)hipGraphExecDestroy(s->exec);
+  if (s->graph) (void)hipGraphDestroy(s->graph);
+  s->exec = nullptr;
+  s->graph = nullptr;
+}
 
-extern "C" {
+// Check the particle arrays of lpm_bve_solver_new / _set_particles.
+int bve_check_particles(lpm_ctx* ctx, const char* who, int n, const double* x, const double* y,
+                        const double* z, const double* relvort, const double* area, const int32_t* active,
+                        const double* u, const double* v, const double* w) {
+  const std::string nm = who;
+  if (n <= 0) return fail(ctx, LPM_ERR_ARG, nm + ": n must be > 0");
+  if (!x || !y || !z || !relvort || !area || !active) return fail(ctx, LPM_ERR_ARG, nm + ": null array argument");
+  if ((u || v || w) && !(u && v && w)) return fail(ctx, LPM_ERR_ARG, nm + ": give all of u, v, w or none");
+  return LPM_OK;
+}
 
-int lpm_bve_solver_new(lpm_ctx* ctx, lpm_bve_solver** out, int n, const double* x, const double* y,
-                       const double* z, const double* relvort, const double* absvort,
-                       const double* area, const int32_t* active, const double* u, const double* v,
-                       const double* w, double radius, double rotation_rate, int precision,
-                       int flags) {
-  if (!ctx || !out) return LPM_ERR_ARG;
-  ctx->err.clear();
-  *out = nullptr;
-  if (n <= 0) return fail(ctx, LPM_ERR_ARG, "lpm_bve_solver_new: n must be > 0");
-  if (!x || !y || !z || !relvort || !area || !active)
-    return fail(ctx, LPM_ERR_ARG, "lpm_bve_solver_new: null array argument");
-  if ((u || v || w) && !(u && v && w))
-    return fail(ctx, LPM_ERR_ARG, "lpm_bve_solver_new: give all of u, v, w or none");
-  if (precision != LPM_PREC_FP64 && precision != LPM_PREC_FP32_KAHAN)
-    return fail(ctx, LPM_ERR_ARG, "lpm_bve_solver_new: bad precision");
-  if (!(radius > 0.0)) return fail(ctx, LPM_ERR_ARG, "lpm_bve_solver_new: radius must be > 0");
-  LPM_HIP(ctx, hipSetDevice(ctx->device));
-  lpm_bve_solver* s = new lpm_bve_solver();
-  s->ctx = ctx;
+// Load a particle set (at creation, and after a remesh): state arrays,
+// stage workspaces (grown, never shrunk), the active-source list, the
+// pointer tables of the RK4 combine, and the velocity when none is given
+// (SetVelocityOnMesh after a remesh, SsrfpackRemesh.f90:552).
+int bve_load(lpm_bve_solver* s, int n, const double* x, const double* y, const double* z, const double* relvort,
+             const double* absvort, const double* area, const int32_t* active, const double* u, const double* v,
+             const double* w) {
+  lpm_ctx* ctx = s->ctx;
+  drop_graph(s);
+  s->loaded = false;
   s->n = n;
-  s->precision = precision;
-  s->flags = flags;
-  s->radius = radius;
-  s->omega = rotation_rate;
   s->has_abs = absvort != nullptr;
   const size_t nb = (size_t)n * sizeof(double);
   int rc = LPM_OK;
@@ -360,8 +364,8 @@ int lpm_bve_solver_new(lpm_ctx* ctx, lpm_bve_solver** out, int n, const double* 
       if (e != hipSuccess) rc = lpm::hip_fail(ctx, e, "hipStreamSynchronize");
     }
   }
-  if (rc == LPM_OK && (flags & LPM_SOLVER_PROJECT_STAGES)) {
-    hipLaunchKernelGGL(project_positions, dim3((n + 255) / 256), dim3(256), 0, ctx->stream, n, radius,
+  if (rc == LPM_OK && (s->flags & LPM_SOLVER_PROJECT_STAGES)) {
+    hipLaunchKernelGGL(project_positions, dim3((n + 255) / 256), dim3(256), 0, ctx->stream, n, s->radius,
                        s->x.as<double>(), s->y.as<double>(), s->z.as<double>());
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) rc = lpm::hip_fail(ctx, e, "project_positions");
@@ -373,11 +377,58 @@ int lpm_bve_solver_new(lpm_ctx* ctx, lpm_bve_solver** out, int n, const double* 
     hipError_t e = hipStreamSynchronize(ctx->stream);
     if (e != hipSuccess) rc = lpm::hip_fail(ctx, e, "hipStreamSynchronize");
   }
-  if (rc != LPM_OK) {
+  s->loaded = rc == LPM_OK;
+  return rc;
+}
+
+int bve_usable(lpm_bve_solver* s) {
+  if (s->loaded) return LPM_OK;
+  return fail(s->ctx, LPM_ERR_STATE, "solver particles were not loaded (a particle reload failed)");
+}
+
+}  // namespace
+
+extern "C" {
+
+int lpm_bve_solver_new(lpm_ctx* ctx, lpm_bve_solver** out, int n, const double* x, const double* y,
+                       const double* z, const double* relvort, const double* absvort,
+                       const double* area, const int32_t* active, const double* u, const double* v,
+                       const double* w, double radius, double rotation_rate, int precision,
+                       int flags) {
+  if (!ctx || !out) return LPM_ERR_ARG;
+  ctx->err.clear();
+  *out = nullptr;
+  if (int rc = bve_check_particles(ctx, "lpm_bve_solver_new", n, x, y, z, relvort, area, active, u, v, w))
+    return rc;
+  if (precision != LPM_PREC_FP64 && precision != LPM_PREC_FP32_KAHAN)
+    return fail(ctx, LPM_ERR_ARG, "lpm_bve_solver_new: bad precision");
+  if (!(radius > 0.0)) return fail(ctx, LPM_ERR_ARG, "lpm_bve_solver_new: radius must be > 0");
+  LPM_HIP(ctx, hipSetDevice(ctx->device));
+  lpm_bve_solver* s = new lpm_bve_solver();
+  s->ctx = ctx;
+  s->precision = precision;
+  s->flags = flags;
+  s->radius = radius;
+  s->omega = rotation_rate;
+  if (int rc = bve_load(s, n, x, y, z, relvort, absvort, area, active, u, v, w)) {
     lpm_bve_solver_delete(s);
     return rc;
   }
   *out = s;
+  return lpm::collect_timing(ctx);
+}
+
+int lpm_bve_solver_set_particles(lpm_bve_solver* s, int n, const double* x, const double* y, const double* z,
+                                 const double* relvort, const double* absvort, const double* area,
+                                 const int32_t* active, const double* u, const double* v, const double* w) {
+  if (!s) return LPM_ERR_ARG;
+  lpm_ctx* ctx = s->ctx;
+  ctx->err.clear();
+  if (int rc = bve_check_particles(ctx, "lpm_bve_solver_set_particles", n, x, y, z, relvort, area, active, u, v,
+                                   w))
+    return rc;
+  LPM_HIP(ctx, hipSetDevice(ctx->device));
+  if (int rc = bve_load(s, n, x, y, z, relvort, absvort, area, active, u, v, w)) return rc;
   return lpm::collect_timing(ctx);
 }
 
@@ -387,6 +438,7 @@ int lpm_bve_solver_steps(lpm_bve_solver* s, double dt, int nsteps) {
   if (!s) return LPM_ERR_ARG;
   lpm_ctx* ctx = s->ctx;
   ctx->err.clear();
+  if (int rc = bve_usable(s)) return rc;
   if (nsteps < 0) return fail(ctx, LPM_ERR_ARG, "nsteps must be >= 0");
   LPM_HIP(ctx, hipSetDevice(ctx->device));
   for (int k = 0; k < nsteps; ++k)
@@ -400,6 +452,7 @@ int lpm_bve_solver_get_state(lpm_bve_solver* s, double* x, double* y, double* z,
   if (!s) return LPM_ERR_ARG;
   lpm_ctx* ctx = s->ctx;
   ctx->err.clear();
+  if (int rc = bve_usable(s)) return rc;
   LPM_HIP(ctx, hipSetDevice(ctx->device));
   int rc = LPM_OK;
   const DevBuf* src[7] = {&s->x, &s->y, &s->z, &s->zeta, &s->u, &s->v, &s->w};
@@ -414,6 +467,7 @@ int lpm_bve_solver_stream(lpm_bve_solver* s, double* psi_rel, double* psi_abs) {
   if (!s || !psi_rel || !psi_abs) return LPM_ERR_ARG;
   lpm_ctx* ctx = s->ctx;
   ctx->err.clear();
+  if (int rc = bve_usable(s)) return rc;
   LPM_HIP(ctx, hipSetDevice(ctx->device));
   if (int rc = bve_stream(s)) return rc;
   if (int rc = download_array(ctx, psi_rel, s->psi_rel, s->n)) return rc;
@@ -427,6 +481,7 @@ int lpm_bve_solver_diagnostics(lpm_bve_solver* s, double* ke, double* enstrophy,
   if (!s || !ke || !enstrophy || !circulation) return LPM_ERR_ARG;
   lpm_ctx* ctx = s->ctx;
   ctx->err.clear();
+  if (int rc = bve_usable(s)) return rc;
   LPM_HIP(ctx, hipSetDevice(ctx->device));
   double out[3];
   if (int rc = lpm::run_diagnostics(ctx, s->n, s->u.as<double>(), s->v.as<double>(), s->w.as<double>(),
@@ -469,6 +524,7 @@ struct lpm_plane_solver {
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   double graph_dt = 0.0;
+  bool loaded = false;  // false after a failed particle reload
 };
 
 namespace {
@@ -574,29 +630,28 @@ int plane_step(lpm_plane_solver* s, double dt) {
   return LPM_OK;
 }
 
-}  // namespace
+int plane_check_particles(lpm_ctx* ctx, const char* who, int n, const double* x, const double* y,
+                          const double* vort, const double* area, const int32_t* active, const double* u,
+                          const double* v) {
+  const std::string nm = who;
+  if (n <= 0) return fail(ctx, LPM_ERR_ARG, nm + ": n must be > 0");
+  if (!x || !y || !vort || !area || !active) return fail(ctx, LPM_ERR_ARG, nm + ": null array argument");
+  if ((u || v) && !(u && v)) return fail(ctx, LPM_ERR_ARG, nm + ": give both u and v or none");
+  return LPM_OK;
+}
 
-extern "C" {
+int plane_usable(lpm_plane_solver* s) {
+  if (s->loaded) return LPM_OK;
+  return fail(s->ctx, LPM_ERR_STATE, "solver particles were not loaded (a particle reload failed)");
+}
 
-int lpm_plane_solver_new(lpm_ctx* ctx, lpm_plane_solver** out, int n, const double* x,
-                         const double* y, const double* vort, const double* area,
-                         const int32_t* active, const double* u, const double* v, int precision,
-                         int flags) {
-  if (!ctx || !out) return LPM_ERR_ARG;
-  ctx->err.clear();
-  *out = nullptr;
-  if (n <= 0) return fail(ctx, LPM_ERR_ARG, "lpm_plane_solver_new: n must be > 0");
-  if (!x || !y || !vort || !area || !active)
-    return fail(ctx, LPM_ERR_ARG, "lpm_plane_solver_new: null array argument");
-  if ((u || v) && !(u && v)) return fail(ctx, LPM_ERR_ARG, "lpm_plane_solver_new: give both u and v or none");
-  if (precision != LPM_PREC_FP64 && precision != LPM_PREC_FP32_KAHAN)
-    return fail(ctx, LPM_ERR_ARG, "lpm_plane_solver_new: bad precision");
-  LPM_HIP(ctx, hipSetDevice(ctx->device));
-  lpm_plane_solver* s = new lpm_plane_solver();
-  s->ctx = ctx;
+// Load a particle set (at creation, and after a remesh); see bve_load.
+int plane_load(lpm_plane_solver* s, int n, const double* x, const double* y, const double* vort,
+               const double* area, const int32_t* active, const double* u, const double* v) {
+  lpm_ctx* ctx = s->ctx;
+  drop_graph(s);
+  s->loaded = false;
   s->n = n;
-  s->precision = precision;
-  s->flags = flags;
   const size_t nb = (size_t)n * sizeof(double);
   int rc = LPM_OK;
   auto chk = [&](int r) {
@@ -628,11 +683,47 @@ int lpm_plane_solver_new(lpm_ctx* ctx, lpm_plane_solver** out, int n, const doub
     hipError_t e = hipStreamSynchronize(ctx->stream);
     if (e != hipSuccess) rc = lpm::hip_fail(ctx, e, "hipStreamSynchronize");
   }
-  if (rc != LPM_OK) {
+  s->loaded = rc == LPM_OK;
+  return rc;
+}
+
+}  // namespace
+
+extern "C" {
+
+int lpm_plane_solver_new(lpm_ctx* ctx, lpm_plane_solver** out, int n, const double* x,
+                         const double* y, const double* vort, const double* area,
+                         const int32_t* active, const double* u, const double* v, int precision,
+                         int flags) {
+  if (!ctx || !out) return LPM_ERR_ARG;
+  ctx->err.clear();
+  *out = nullptr;
+  if (int rc = plane_check_particles(ctx, "lpm_plane_solver_new", n, x, y, vort, area, active, u, v)) return rc;
+  if (precision != LPM_PREC_FP64 && precision != LPM_PREC_FP32_KAHAN)
+    return fail(ctx, LPM_ERR_ARG, "lpm_plane_solver_new: bad precision");
+  LPM_HIP(ctx, hipSetDevice(ctx->device));
+  lpm_plane_solver* s = new lpm_plane_solver();
+  s->ctx = ctx;
+  s->precision = precision;
+  s->flags = flags;
+  if (int rc = plane_load(s, n, x, y, vort, area, active, u, v)) {
     lpm_plane_solver_delete(s);
     return rc;
   }
   *out = s;
+  return lpm::collect_timing(ctx);
+}
+
+int lpm_plane_solver_set_particles(lpm_plane_solver* s, int n, const double* x, const double* y,
+                                   const double* vort, const double* area, const int32_t* active,
+                                   const double* u, const double* v) {
+  if (!s) return LPM_ERR_ARG;
+  lpm_ctx* ctx = s->ctx;
+  ctx->err.clear();
+  if (int rc = plane_check_particles(ctx, "lpm_plane_solver_set_particles", n, x, y, vort, area, active, u, v))
+    return rc;
+  LPM_HIP(ctx, hipSetDevice(ctx->device));
+  if (int rc = plane_load(s, n, x, y, vort, area, active, u, v)) return rc;
   return lpm::collect_timing(ctx);
 }
 
@@ -642,6 +733,7 @@ int lpm_plane_solver_steps(lpm_plane_solver* s, double dt, int nsteps) {
   if (!s) return LPM_ERR_ARG;
   lpm_ctx* ctx = s->ctx;
   ctx->err.clear();
+  if (int rc = plane_usable(s)) return rc;
   if (nsteps < 0) return fail(ctx, LPM_ERR_ARG, "nsteps must be >= 0");
   LPM_HIP(ctx, hipSetDevice(ctx->device));
   for (int k = 0; k < nsteps; ++k)
@@ -654,6 +746,7 @@ int lpm_plane_solver_get_state(lpm_plane_solver* s, double* x, double* y, double
   if (!s) return LPM_ERR_ARG;
   lpm_ctx* ctx = s->ctx;
   ctx->err.clear();
+  if (int rc = plane_usable(s)) return rc;
   LPM_HIP(ctx, hipSetDevice(ctx->device));
   const DevBuf* src[4] = {&s->x, &s->y, &s->u, &s->v};
   double* dst[4] = {x, y, u, v};
@@ -667,6 +760,7 @@ int lpm_plane_solver_stream(lpm_plane_solver* s, double* psi) {
   if (!s || !psi) return LPM_ERR_ARG;
   lpm_ctx* ctx = s->ctx;
   ctx->err.clear();
+  if (int rc = plane_usable(s)) return rc;
   LPM_HIP(ctx, hipSetDevice(ctx->device));
   if (int rc = plane_stream(s)) return rc;
   if (int rc = download_array(ctx, psi, s->psi, s->n)) return rc;
@@ -679,6 +773,7 @@ int lpm_plane_solver_diagnostics(lpm_plane_solver* s, double* ke, double* enstro
   if (!s || !ke || !enstrophy || !circulation) return LPM_ERR_ARG;
   lpm_ctx* ctx = s->ctx;
   ctx->err.clear();
+  if (int rc = plane_usable(s)) return rc;
   LPM_HIP(ctx, hipSetDevice(ctx->device));
   double out[3];
   if (int rc = lpm::run_diagnostics(ctx, s->n, s->u.as<double>(), s->v.as<double>(), nullptr,

@@ -88,7 +88,9 @@ _SIGS = {
     "lpm_bve_solver_stream": ([_vp, _pd, _pd], _i),
     "lpm_bve_solver_diagnostics": ([_vp, _pd, _pd, _pd], _i),
     "lpm_bve_solver_delete": ([_vp], _i),
+    "lpm_bve_solver_set_particles": ([_vp, _i, _pd, _pd, _pd, _pd, _pd, _pd, _pi, _pd, _pd, _pd], _i),
     "lpm_plane_solver_new": ([_vp, ctypes.POINTER(_vp), _i, _pd, _pd, _pd, _pd, _pi, _pd, _pd, _i, _i], _i),
+    "lpm_plane_solver_set_particles": ([_vp, _i, _pd, _pd, _pd, _pd, _pi, _pd, _pd], _i),
     "lpm_plane_solver_timestep": ([_vp, _d], _i),
     "lpm_plane_solver_steps": ([_vp, _d, _i], _i),
     "lpm_plane_solver_get_state": ([_vp, _pd, _pd, _pd, _pd], _i),

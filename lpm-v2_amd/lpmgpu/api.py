@@ -310,6 +310,19 @@ class BVESolver:
         self.handle = h
         ctx.adopt(self)
 
+    def set_particles(self, x, y, z, relvort, area, active, absvort=None, u=None, v=None, w=None):
+        """Remesh re-upload (lpm_bve_solver_set_particles): a new particle set of any size; the
+        velocity is recomputed on the device unless u, v, w are given."""
+        arrs = [f64(a) for a in (x, y, z, relvort, area)]
+        act = i32(active)
+        absv = None if absvort is None else f64(absvort)
+        uvw = [None if a is None else f64(a) for a in (u, v, w)]
+        n = arrs[0].shape[0]
+        self.ctx.check(self.ctx._lib.lpm_bve_solver_set_particles(
+            self.handle, n, pd(arrs[0]), pd(arrs[1]), pd(arrs[2]), pd(arrs[3]), pd(absv), pd(arrs[4]), pi(act),
+            *[pd(a) for a in uvw]))
+        self.n = n
+
     def timestep(self, dt: float, nsteps: int = 1):
         self.ctx.check(self.ctx._lib.lpm_bve_solver_steps(self.handle, dt, nsteps))
 
@@ -359,6 +372,16 @@ class PlaneSolver:
                                                 prec_code(precision), flags))
         self.handle = h
         ctx.adopt(self)
+
+    def set_particles(self, x, y, vort, area, active, u=None, v=None):
+        """Remesh re-upload (lpm_plane_solver_set_particles)."""
+        arrs = [f64(a) for a in (x, y, vort, area)]
+        act = i32(active)
+        uv = [None if a is None else f64(a) for a in (u, v)]
+        n = arrs[0].shape[0]
+        self.ctx.check(self.ctx._lib.lpm_plane_solver_set_particles(
+            self.handle, n, pd(arrs[0]), pd(arrs[1]), pd(arrs[2]), pd(arrs[3]), pi(act), *[pd(a) for a in uv]))
+        self.n = n
 
     def timestep(self, dt: float, nsteps: int = 1):
         self.ctx.check(self.ctx._lib.lpm_plane_solver_steps(self.handle, dt, nsteps))

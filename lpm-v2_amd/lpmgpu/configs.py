@@ -176,3 +176,97 @@ def run_cfg3(ctx: Context, steps=None, autotune=True, log=print):
     b64.delete()
     out["finite_after_steps"] = bool(all(np.isfinite(s["u"]).all() for s in st))
     return out
+
+
+# Algorithmic flops per pair of the PSE operator and SWE kernels (+, -, x = 1,
+# fma = 2, exp / rcp = 1), counted from the per-pair arithmetic of each:
+#   plane derivative sums, M fields: separation 2, r^2 3, exp 1, kernel cubic 6,
+#     x exp, x A 2, X_k g 2, M_k 2, c_m X_k 4M        -> 18 + 4 M
+#   sphere derivative sums (series distance): separation 3, chord^2 5, u 1,
+#     theta^2 series 13, exp 1, cubic 6, x exp, x A 2, X 3, M 3, fields 6M -> 37 + 6 M
+#   interpolation: plane 15, sphere (series) 32
+#   SWE right-hand side: velocity 10 + 1 rcp, gradient 20, surface Laplacian 13, r^2 and
+#     separation 5, strengths 2                                                  -> 51
+OPS_FLOPS = {"plane_grad": 22, "plane_m2": 26, "sphere_grad": 43, "sphere_m3": 55,
+             "interp_plane": 15, "interp_sphere": 32, "swe": 51}
+
+
+def run_pse_ops(ctx: Context, log=print, plane_side=512, sphere_level=7):
+    """PSE derivative operators, interpolation and the SWE right-hand side at
+    full size: the 512^2 jittered grid (eps = 2h) and icosahedral level 7
+    (eps = h^0.75, the PSE default, which takes the series distance).  FP32
+    compensated, timed by the library's HIP events, against the FP64 GPU sums.
+    (The grid's jitter, a quarter of the spacing, is large for the reference's
+    (c_j + c_i) quadrature, so the operators' own accuracy is shown by the
+    oracle pins, not here.)"""
+    w = workloads.cfg2(plane_side)
+    x, y, area, eps = w["x"], w["y"], w["area"], w["eps"]
+    n = x.shape[0]
+    act = np.ones(n, np.int32)
+    r2 = x * x + y * y
+    f = w["f"]
+    res = {"workload": f"PSE operators and SWE right-hand side: {plane_side}^2 jittered grid (eps = 2h), "
+                       f"icosahedral L{sphere_level} (eps = h^0.75)",
+           "plane_particles": n}
+    ctx.set_timing(True)
+
+    def timed(name, kind, flops, pairs, call32, call64):
+        ctx.reset_kernel_stats()
+        t0 = time.perf_counter()
+        g32 = call32()
+        t32 = time.perf_counter() - t0
+        k = _kernel(ctx, kind, flops, pairs)
+        g64 = call64()
+        r = {"fp32_call_ms": round(1e3 * t32, 2), "pair_kernel": k, "flops_per_pair": flops, "pairs": pairs,
+             "fp32_vs_fp64": _err(g32, g64)}
+        res[name] = r
+        log(f"[{name}] {r}")
+
+    pairs = n * n
+    timed("plane_gradient", L.K_PSE_DERIV_PLANE, OPS_FLOPS["plane_grad"], pairs,
+          lambda: ctx.pse_plane_gradient(x, y, f, area, act, eps, precision="fp32"),
+          lambda: ctx.pse_plane_gradient(x, y, f, area, act, eps, precision="fp64"))
+    u, v = np.sin(2 * y) + x, np.cos(x) * y
+    timed("plane_double_dot", L.K_PSE_DERIV_PLANE, OPS_FLOPS["plane_m2"], pairs,
+          lambda: ctx.pse_plane_double_dot(x, y, u, v, area, act, eps, precision="fp32"),
+          lambda: ctx.pse_plane_double_dot(x, y, u, v, area, act, eps, precision="fp64"))
+    rng = np.random.default_rng(20261202)
+    px, py = rng.uniform(-0.7, 0.7, n), rng.uniform(-0.7, 0.7, n)
+    timed("plane_interpolation", L.K_PSE_INTERP_PLANE, OPS_FLOPS["interp_plane"], pairs,
+          lambda: ctx.pse_plane_interpolate(x, y, f, area, act, eps, px, py, precision="fp32"),
+          lambda: ctx.pse_plane_interpolate(x, y, f, area, act, eps, px, py, precision="fp64"))
+    vort = 3.0 * np.exp(-4 * r2)
+    div = 0.5 * np.sin(2 * x) * np.cos(y)
+    hgt = 1.0 + 0.1 * np.exp(-6 * ((x - 0.2) ** 2 + y * y))
+    bottom = 0.05 * x - 0.02 * y * y
+    timed("swe_plane_rhs", L.K_SWE_PLANE, OPS_FLOPS["swe"], pairs - n,
+          lambda: ctx.swe_plane_rhs(x, y, vort, div, hgt, bottom, area, act, eps, precision="fp32")[:3],
+          lambda: ctx.swe_plane_rhs(x, y, vort, div, hgt, bottom, area, act, eps, precision="fp64")[:3])
+
+    from . import mesh as _mesh
+    m = _mesh.icosahedral_mesh(sphere_level)
+    seps = float(m.max_edge_length ** 0.75)
+    lat = np.arcsin(np.clip(m.z, -1, 1))
+    lon = np.arctan2(m.y, m.x)
+    s = np.sin(lat)
+    fs = 3.0 * np.sqrt(35.0) * np.cos(4 * lon) * s * (s * s - 1.0) ** 2
+    S = int(m.active.sum())
+    res["sphere_targets"], res["sphere_sources"], res["sphere_eps"] = m.n, S, seps
+    timed("sphere_gradient", L.K_PSE_DERIV_SPHERE, OPS_FLOPS["sphere_grad"], m.n * S,
+          lambda: ctx.pse_sphere_gradient(m.x, m.y, m.z, fs, m.area, m.active, seps, precision="fp32"),
+          lambda: ctx.pse_sphere_gradient(m.x, m.y, m.z, fs, m.area, m.active, seps, precision="fp64"))
+    g = ctx.pse_sphere_gradient(m.x, m.y, m.z, fs, m.area, m.active, seps, precision="fp64")
+    timed("sphere_divergence", L.K_PSE_DERIV_SPHERE, OPS_FLOPS["sphere_m3"], m.n * S,
+          lambda: ctx.pse_sphere_divergence(m.x, m.y, m.z, *g, m.area, m.active, seps, precision="fp32"),
+          lambda: ctx.pse_sphere_divergence(m.x, m.y, m.z, *g, m.area, m.active, seps, precision="fp64"))
+    lats = -0.5 * np.pi + np.arange(181) * np.pi / 180.0
+    lons = np.arange(360) * np.pi / 180.0
+    LA, LO = np.meshgrid(lats, lons, indexing="ij")
+    qx, qy, qz = (np.cos(LO) * np.cos(LA)).ravel(), (np.sin(LO) * np.cos(LA)).ravel(), np.sin(LA).ravel()
+    timed("sphere_interpolation_latlon", L.K_PSE_INTERP_SPHERE, OPS_FLOPS["interp_sphere"], qx.size * S,
+          lambda: ctx.pse_sphere_interpolate(m.x, m.y, m.z, fs, m.area, m.active, seps, qx, qy, qz,
+                                             precision="fp32"),
+          lambda: ctx.pse_sphere_interpolate(m.x, m.y, m.z, fs, m.area, m.active, seps, qx, qy, qz,
+                                             precision="fp64"))
+    ctx.set_timing(False)
+    return res

@@ -26,6 +26,7 @@ L.extra_signatures({
     "lpm_bve_ff_steps": ([_vp, _d, _i, _i, _i, _d, ctypes.POINTER(_i), L._pd, ctypes.POINTER(_i)], _i),
     "lpm_bve_ff_get_state": ([_vp, L._pd, L._pd, L._pd, L._pd, L._pd, L._pd, L._pd], _i),
     "lpm_bve_ff_set_state": ([_vp, L._pd, L._pd, L._pd, L._pd], _i),
+    "lpm_bve_ff_set_particles": ([_vp, _i, L._pd, L._pd, L._pd, L._pd, L._pd, L._pi], _i),
     "lpm_bve_ff_cells": ([_vp, L._pi, L._pi], _i),
     "lpm_bve_ff_particle_cells": ([_vp, L._pi], _i),
     "lpm_bve_ff_delete": ([_vp], _i),
@@ -122,6 +123,16 @@ class FarFieldBVE:
     def set_state(self, x, y, z, relvort):
         arr = [f64(a) for a in (x, y, z, relvort)]
         self.ctx.check(self.ctx._lib.lpm_bve_ff_set_state(self.handle, *[pd(a) for a in arr]))
+
+    def set_particles(self, x, y, z, relvort, area, active):
+        """Remesh re-upload (lpm_bve_ff_set_particles): a new particle set of any size; cells,
+        the trained network and the collected samples are kept."""
+        arr = [f64(a) for a in (x, y, z, relvort, area)]
+        act = np.ascontiguousarray(active, dtype=np.int32)
+        n = arr[0].shape[0]
+        self.ctx.check(self.ctx._lib.lpm_bve_ff_set_particles(self.handle, n, *[pd(a) for a in arr],
+                                                              act.ctypes.data_as(L._pi)))
+        self.n = n
 
     def pretrain(self, dt, states=5, every=5, samples=2_000_000, epochs=10, batch=4096, lr=1e-3, seed=1):
         """Train on direct-sum samples of the method's own run: collect at
