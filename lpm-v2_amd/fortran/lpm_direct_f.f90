@@ -230,6 +230,36 @@ module lpm_direct_f
       real(c_double), value :: dt
     end function
 
+    ! remesh re-upload (SsrfpackRemesh.f90:388-, :552); absvort and u, v, w may be c_null_ptr
+    integer(c_int) function lpm_bve_solver_set_particles(solver, n, x, y, z, relvort, absvort, area, active, &
+        u, v, w) bind(C, name="lpm_bve_solver_set_particles")
+      import :: c_int, c_ptr, c_double, c_int32_t
+      type(c_ptr), value :: solver
+      integer(c_int), value :: n
+      real(c_double), intent(in) :: x(*), y(*), z(*), relvort(*), area(*)
+      type(c_ptr), value :: absvort, u, v, w
+      integer(c_int32_t), intent(in) :: active(*)
+    end function
+
+    integer(c_int) function lpm_plane_solver_set_particles(solver, n, x, y, vort, area, active, u, v) &
+        bind(C, name="lpm_plane_solver_set_particles")
+      import :: c_int, c_ptr, c_double, c_int32_t
+      type(c_ptr), value :: solver
+      integer(c_int), value :: n
+      real(c_double), intent(in) :: x(*), y(*), vort(*), area(*)
+      integer(c_int32_t), intent(in) :: active(*)
+      type(c_ptr), value :: u, v
+    end function
+
+    integer(c_int) function lpm_bve_ff_set_particles(ff, n, x, y, z, relvort, area, active) &
+        bind(C, name="lpm_bve_ff_set_particles")
+      import :: c_int, c_ptr, c_double, c_int32_t
+      type(c_ptr), value :: ff
+      integer(c_int), value :: n
+      real(c_double), intent(in) :: x(*), y(*), z(*), relvort(*), area(*)
+      integer(c_int32_t), intent(in) :: active(*)
+    end function
+
     integer(c_int) function lpm_bve_solver_get_state(solver, x, y, z, relvort, u, v, w) &
         bind(C, name="lpm_bve_solver_get_state")
       import :: c_int, c_ptr, c_double
