@@ -418,9 +418,11 @@ int run_pair_sum(lpm_ctx* ctx, const PairProblem& pp) {
   const int no = ops ? ops_nout(kind, pp.m) : nout_of(kind);
   const int nw = ops ? ops_record_words(kind, pp.m) : SREC;
   if (ops && !ov.r) {
-    // up to 12 sums per target: fewer targets per lane keep the registers
+    // targets per lane by the number of sums, from a sweep of R in {1, 2, 4} on
+    // MI355X (512^2 plane, icosahedral L7): 1 sum -> 4, 4 or 6 -> 2, 7 (SWE) -> 1,
+    // 12 (sphere, three fields) -> 4 (fewer waves, but the source loads amortise)
     if (prec == LPM_PREC_FP32_KAHAN)
-      tu.r = no > 4 ? 2 : 4;
+      tu.r = no == 1 || no == 12 ? 4 : (no == 7 ? 1 : 2);
     else
       tu.r = no > 4 ? 1 : 2;
   }
