@@ -88,6 +88,64 @@ def cpu_baseline(m, zeta, target_seconds: float, max_threads: int):
     }
 
 
+def pse_ops_cpu_baseline(res: dict, max_threads: int, plane_block: int = 2048, sphere_block: int = 1024):
+    """FP64 oracle (the reference loops restated in C, OpenMP over a contiguous
+    target block) for each operator of the pse_ops workload, on the same inputs
+    and a bounded block of targets; adds a cpu_baseline entry (G pairs/s) to
+    each operator's result."""
+    from lpmgpu import mesh, workloads
+    from oracle import oracle
+
+    threads = oracle.set_threads(max_threads)
+    w = workloads.cfg2(512)
+    x, y, area, eps, f = w["x"], w["y"], w["area"], w["eps"], w["f"]
+    n = x.shape[0]
+    act = np.ones(n, np.int32)
+    u, v = np.sin(2 * y) + x, np.cos(x) * y
+    r2 = x * x + y * y
+    vort, div = 3.0 * np.exp(-4 * r2), 0.5 * np.sin(2 * x) * np.cos(y)
+    hgt, bottom = 1.0 + 0.1 * np.exp(-6 * ((x - 0.2) ** 2 + y * y)), 0.05 * x - 0.02 * y * y
+    rng = np.random.default_rng(20261202)
+    px, py = rng.uniform(-0.7, 0.7, n), rng.uniform(-0.7, 0.7, n)
+    m = mesh.icosahedral_mesh(7)
+    seps = float(m.max_edge_length ** 0.75)
+    S = int(m.active.sum())
+    fs = np.sin(3 * m.z) + m.x
+    g = (m.y, -m.x, 0.0 * m.z)
+    lats = -0.5 * np.pi + np.arange(181) * np.pi / 180.0
+    lons = np.arange(360) * np.pi / 180.0
+    LA, LO = np.meshgrid(lats, lons, indexing="ij")
+    qx, qy, qz = (np.cos(LO) * np.cos(LA)).ravel(), (np.sin(LO) * np.cos(LA)).ravel(), np.sin(LA).ravel()
+    bp, bs = plane_block, sphere_block
+    cases = {
+        "plane_gradient": (lambda: oracle.pse_plane_gradient(x, y, f, area, act, eps, 0, bp), bp * n),
+        "plane_double_dot": (lambda: oracle.pse_plane_double_dot(x, y, u, v, area, act, eps, 0, bp), bp * n),
+        "plane_interpolation": (lambda: oracle.pse_plane_interpolate(x, y, f, area, act, eps, px[:bp], py[:bp]),
+                                bp * n),
+        "swe_plane_rhs": (lambda: oracle.swe_plane_rhs(x, y, vort, div, hgt, bottom, area, act, eps, 0, bp),
+                          bp * (n - 1)),
+        "sphere_gradient": (lambda: oracle.pse_sphere_gradient(m.x, m.y, m.z, fs, m.area, m.active, seps, 1.0, 0, bs),
+                            bs * S),
+        "sphere_divergence": (lambda: oracle.pse_sphere_divergence(m.x, m.y, m.z, *g, m.area, m.active, seps, 1.0,
+                                                                   0, bs), bs * S),
+        "sphere_interpolation_latlon": (lambda: oracle.pse_sphere_interpolate(
+            m.x, m.y, m.z, fs, m.area, m.active, seps, qx[:bs], qy[:bs], qz[:bs]), bs * S),
+    }
+    for name, (call, pairs) in cases.items():
+        t0 = time.perf_counter()
+        call()
+        dt = time.perf_counter() - t0
+        entry = {"value": pairs / dt / 1e9, "unit": "G pair-interactions/s", "cores": threads, "kind": "port",
+                 "sample": f"FP64 oracle, first {bp if name.startswith(('plane', 'swe')) else bs} targets "
+                           f"({pairs / 1e9:.2f} G pairs, {dt:.1f} s)"}
+        if name in res:
+            res[name]["cpu_baseline"] = entry
+            k = res[name].get("pair_kernel") or {}
+            if k.get("avg_ms"):
+                res[name]["gpu_vs_cpu"] = round(res[name]["pairs"] / (k["avg_ms"] * 1e-3) / 1e9 / entry["value"], 1)
+        log(f"[cpu {name}] {entry}")
+
+
 def learned_far_field(ctx, m, zeta, omega, dt, plain_ms_per_step, steps, samples, epochs):
     """The learned far-field form on the same workload (1,280 level-3 cells, near
     radius 0.15, network band to 0.30, 4th-order cell expansion beyond): train on
@@ -202,6 +260,8 @@ def main():
               "cfg3": configs.run_cfg3, "pse_ops": configs.run_pse_ops}[args.workload]
         kw = {"steps": args.cfg_steps} if args.cfg_steps and args.workload in ("cfg1", "cfg3") else {}
         res = fn(ctx, log=log, **kw)
+        if args.workload == "pse_ops" and args.cpu_seconds > 0:
+            pse_ops_cpu_baseline(res, args.cpu_threads)
         res["metric"] = METRIC
         print(json.dumps(res), flush=True)
         ctx.close()
