@@ -3,7 +3,13 @@
 !> (int32 n, then x, y, z, relVort, area as float64[n], then int32 active[n]),
 !> calls lpm_BVESphereVelocity over two MPISetup-style blocks plus one RK4
 !> step of GPUBVESolver, and writes u, v, w then the stepped x, y, z, relVort
-!> (float64[n] each) to the output file.
+!> (float64[n] each) to the output file.  It then calls the PSE and SWE entry
+!> points through their bind(C) interfaces (FP64): the sphere PSE gradient
+!> (eps 0.3), the planar SWE right-hand side on (x, y) (eps 0.3), the sphere
+!> interpolation of relVort at the first 10 particles, and a solver reloaded
+!> with the same particles and stepped once; it appends gx, gy, gz, u, v, ddot,
+!> lapSurf (n each), the 10 interpolated values and the reloaded solver's
+!> stepped x, y, z, relVort (n each).
 program lpm_fortran_smoke
   use iso_c_binding
   use lpm_direct_f
@@ -15,8 +21,11 @@ program lpm_fortran_smoke
   logical(4), allocatable :: mask(:)
   real(c_double), allocatable, target :: x(:), y(:), z(:), zeta(:), area(:), u(:), v(:), w(:)
   real(c_double), allocatable :: xs(:), ys(:), zs(:), zetas(:), us(:), vs(:), ws(:)
+  real(c_double), allocatable :: gx(:), gy(:), gz(:), su(:), sv(:), sdd(:), slap(:), dv(:), hh(:), bot(:)
+  real(c_double) :: pint(10)
   type(GPUBVESolver) :: solver
   integer :: half, unit_in, unit_out
+  integer(c_int) :: rc
 
   call get_command_argument(1, inFile)
   call get_command_argument(2, outFile)
@@ -44,6 +53,34 @@ program lpm_fortran_smoke
 
   open(newunit=unit_out, file=trim(outFile), access='stream', form='unformatted', status='replace')
   write(unit_out) u, v, w, xs, ys, zs, zetas
+
+  ! PSE and SWE entry points (the context was created by the calls above)
+  allocate(gx(n), gy(n), gz(n), su(n), sv(n), sdd(n), slap(n), dv(n), hh(n), bot(n))
+  gx = 0.0_c_double; gy = 0.0_c_double; gz = 0.0_c_double
+  rc = lpm_pse_sphere_gradient(lpm_context, gx, gy, gz, x, y, z, zeta, area, act, n, 0_c_int, n, &
+                               0.3_c_double, 1.0_c_double, LPM_PREC_FP64)
+  if (rc /= LPM_OK) print *, "lpm_pse_sphere_gradient ERROR ", rc
+  dv = 0.1_c_double * zeta
+  hh = 1.0_c_double + 0.1_c_double * z
+  bot = 0.05_c_double * x
+  rc = lpm_swe_plane_rhs(lpm_context, su, sv, sdd, slap, x, y, zeta, dv, hh, bot, area, act, n, 0_c_int, n, &
+                         0.3_c_double, LPM_PREC_FP64)
+  if (rc /= LPM_OK) print *, "lpm_swe_plane_rhs ERROR ", rc
+  rc = lpm_pse_sphere_interpolate(lpm_context, pint, x(1:10), y(1:10), z(1:10), 10_c_int, x, y, z, zeta, area, &
+                                  act, n, 0.3_c_double, 1.0_c_double, LPM_PREC_FP64)
+  if (rc /= LPM_OK) print *, "lpm_pse_sphere_interpolate ERROR ", rc
+  write(unit_out) gx, gy, gz, su, sv, sdd, slap, pint
+
+  ! remesh re-upload: a solver reloaded with the same particles steps like a new one
+  call New(solver, x, y, z, zeta, area, mask, u, v, w, 1.0_c_double, 2.0_c_double * acos(-1.0_c_double), &
+           LPM_PREC_FP64, 0)
+  rc = lpm_bve_solver_set_particles(solver%handle, n, x, y, z, zeta, c_null_ptr, area, act, &
+                                    c_loc(u), c_loc(v), c_loc(w))
+  if (rc /= LPM_OK) print *, "lpm_bve_solver_set_particles ERROR ", rc
+  call Timestep(solver, 0.01_c_double)
+  call GetState(solver, xs, ys, zs, zetas, us, vs, ws)
+  call Delete(solver)
+  write(unit_out) xs, ys, zs, zetas
   close(unit_out)
   print *, "lpm_fortran_smoke: n =", n, " done"
 end program

@@ -30,10 +30,26 @@ def test_fortran_drop_in(oracle, tmp_path):
     r = subprocess.run([EXE, str(inp), str(out)], capture_output=True, text=True, timeout=120)
     print(r.stdout, r.stderr)
     assert r.returncode == 0 and "ERROR" not in r.stdout
-    res = np.fromfile(out, np.float64).reshape(7, n)
+    raw = np.fromfile(out, np.float64)
+    assert raw.size == 18 * n + 10
+    res = raw[:7 * n].reshape(7, n)
     ref = oracle.bve_sphere_velocity(m.x, m.y, m.z, zeta, m.area, m.active)
     assert rel_err_max(tuple(res[:3]), ref) < 1e-11
     st = dict(x=m.x, y=m.y, z=m.z, relvort=zeta, u=ref[0], v=ref[1], w=ref[2])
     o = oracle.bve_timestep(st, m.area, m.active, 1.0, 2 * np.pi, 0.01)
     assert rel_err_max(tuple(res[3:6]), (o["x"], o["y"], o["z"])) < 1e-11
     assert rel_err_max(res[6], o["relvort"]) < 1e-11
+
+    # PSE / SWE entry points through their bind(C) interfaces
+    ops = raw[7 * n: 14 * n].reshape(7, n)
+    g = oracle.pse_sphere_gradient(m.x, m.y, m.z, zeta, m.area, m.active, 0.3)
+    assert rel_err_max(tuple(ops[:3]), g) < 1e-11
+    swe = oracle.swe_plane_rhs(m.x, m.y, zeta, 0.1 * zeta, 1.0 + 0.1 * m.z, 0.05 * m.x, m.area, m.active, 0.3)
+    for k in range(4):
+        assert rel_err_max(ops[3 + k], swe[k]) < 1e-11, k
+    pint = raw[14 * n: 14 * n + 10]
+    ip = oracle.pse_sphere_interpolate(m.x, m.y, m.z, zeta, m.area, m.active, 0.3, m.x[:10], m.y[:10], m.z[:10])
+    assert rel_err_max(pint, ip) < 1e-11
+    # the reloaded solver's step equals the first solver's, bit for bit
+    reload = raw[14 * n + 10:].reshape(4, n)
+    assert np.array_equal(reload, res[3:7])
