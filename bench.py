@@ -42,7 +42,7 @@ PEAK_HBM_GBS = 8000.0
 # HBM bytes per pair-kernel launch at level 8 from rocprofv3 FETCH_SIZE and
 # WRITE_SIZE runs, scaled by the same counters on a 1 GiB device copy (factors
 # 2.0 and 1.0): profiles/sphere_bs_fp32_icos8_hbm_counters.md
-MEASURED_TRAFFIC_BYTES = {8: 8.88e9}
+MEASURED_TRAFFIC_BYTES = {8: 4.22e9}
 
 
 def log(*a):

@@ -221,15 +221,24 @@ static int nout_of(int kind) {
 Tuning default_tuning(int kind, int prec) {
   Tuning t;
   if (prec == LPM_PREC_FP32_KAHAN) {
-    // measured on MI355X at icosahedral L7/L8: R = 4 beats 2 and 8 (occupancy
-    // 6 waves/SIMD at 74 VGPRs), longer compensated tiles cost less
-    t.r = 4;
+    // targets per lane from a sweep of R in {1, 2, 4, 8} x KT in {64 ... 512}
+    // on MI355X (icosahedral L7/L8, 512^2 plane, source chunks of ~2,048):
+    // the Biot-Savart and sphere Green sums run fastest at R = 4, the plane
+    // PSE sums at R = 2, the plane Green sum at R = 8 and the sphere PSE sum
+    // (long series distance, more registers) at R = 1; KT beyond 256 gains
+    // under 1 %
+    switch (kind) {
+      case LPM_K_PSE8:
+      case LPM_K_PSE2: t.r = 2; break;
+      case LPM_K_PLANE_GREEN: t.r = 8; break;
+      case LPM_K_PSE_SPHERE8: t.r = 1; break;
+      default: t.r = 4; break;
+    }
     t.kt = 256;
   } else {
     t.r = 2;
     t.kt = 64;
   }
-  (void)kind;
   return t;
 }
 
@@ -347,28 +356,41 @@ int blocks_per_cu(const void* kfn) {
   return nb;
 }
 
-// Source splits: fill the chip in whole "rounds" of resident blocks.  All
-// blocks of a launch carry equal work, so the launch time is
-// ceil(B / slots) rounds; pick the fewest splits whose round efficiency
-// B / (rounds * slots) is at least 0.9 (or the best one available).
-static int choose_splits(int n_wtiles, int ns, int kt, int slots) {
+// Source splits.  Short source chunks are faster on MI355X: a sweep of the
+// chunk length (1,024 ... 32,768 sources, and the whole-chip-round choice
+// below) over the sphere, plane, Green, PSE and SWE kernels, FP32 and FP64,
+// at 10^5-2*10^6 targets, put the fastest launches at 1,024-2,048 sources per
+// block (7-20 % faster than chunks of 10^4-10^5), from finer dynamic balance
+// across the 256 CUs and a source working set that stays in L2.  So: chunks of
+// kChunkTarget sources, and at least enough splits that the blocks fill the
+// chip in whole rounds of resident blocks (round efficiency B / (rounds *
+// slots) >= 0.9, for small problems), within max_split_mem splits (the FP64
+// partials of all splits are kept until the ordered reduction).
+constexpr int kChunkTarget = 2048;
+constexpr size_t kPartialCap = size_t(2) << 30;  // bytes of FP64 split partials
+
+static int choose_splits(int n_wtiles, int ns, int kt, int slots, int max_split_mem) {
   const int nblk_t = (n_wtiles + 3) / 4;
   const int min_chunk = std::max(kt, 256);
   int max_split = std::max(1, ns / min_chunk);
-  max_split = std::min(max_split, 4096);
-  int best = 1;
+  max_split = std::min(max_split, std::max(1, max_split_mem));
+  int fill = 1;
   double best_eff = -1.0;
   for (int s = 1; s <= max_split; ++s) {
     const double B = (double)nblk_t * s;
     const double rounds = std::ceil(B / slots);
     const double eff = B / (rounds * slots);
-    if (eff >= 0.9) return s;
+    if (eff >= 0.9) {
+      fill = s;
+      break;
+    }
     if (eff > best_eff + 1e-9) {
       best_eff = eff;
-      best = s;
+      fill = s;
     }
   }
-  return best;
+  const int by_chunk = (ns + kChunkTarget - 1) / kChunkTarget;
+  return std::max(1, std::min(max_split, std::max(fill, by_chunk)));
 }
 
 hipEvent_t take_event(lpm_ctx* ctx) {
@@ -418,11 +440,12 @@ int run_pair_sum(lpm_ctx* ctx, const PairProblem& pp) {
   const int no = ops ? ops_nout(kind, pp.m) : nout_of(kind);
   const int nw = ops ? ops_record_words(kind, pp.m) : SREC;
   if (ops && !ov.r) {
-    // targets per lane by the number of sums, from a sweep of R in {1, 2, 4} on
-    // MI355X (512^2 plane, icosahedral L7): 1 sum -> 4, 4 or 6 -> 2, 7 (SWE) -> 1,
-    // 12 (sphere, three fields) -> 4 (fewer waves, but the source loads amortise)
+    // targets per lane from a sweep of R in {1, 2, 4, 8} on MI355X (512^2
+    // plane, icosahedral L7): the sphere operators (series distance) at R = 1,
+    // the plane derivative sums and interpolation at R = 2, the 7-sum SWE
+    // kernel at R = 1
     if (prec == LPM_PREC_FP32_KAHAN)
-      tu.r = no == 1 || no == 12 ? 4 : (no == 7 ? 1 : 2);
+      tu.r = kind == LPM_K_PSE_DERIV_SPHERE || kind == LPM_K_PSE_INTERP_SPHERE || kind == LPM_K_SWE_PLANE ? 1 : 2;
     else
       tu.r = no > 4 ? 1 : 2;
   }
@@ -442,8 +465,10 @@ int run_pair_sum(lpm_ctx* ctx, const PairProblem& pp) {
   if (!fn) return fail(ctx, LPM_ERR_ARG, "no pair kernel for this kind, precision and field count");
   int nsplit = 1, chunk = std::max(ns, 1);
   if (ns > 0) {
+    const int split_mem = (int)std::min<size_t>(4096, kPartialCap / ((size_t)no * nt * sizeof(double)));
     int want = tu.splits > 0 ? tu.splits
-                             : choose_splits(n_wtiles, ns, tu.kt, blocks_per_cu((const void*)fn) * ctx->num_cus);
+                             : choose_splits(n_wtiles, ns, tu.kt, blocks_per_cu((const void*)fn) * ctx->num_cus,
+                                             split_mem);
     chunk = (ns + want - 1) / want;
     chunk = (chunk + tu.kt - 1) / tu.kt * tu.kt;
     nsplit = (ns + chunk - 1) / chunk;

@@ -194,7 +194,13 @@ KernelFn pick_r(int r) {
   switch (r) {
     case 1: return pair_sum_kernel<P, T, 1, C>;
     case 2: return pair_sum_kernel<P, T, 2, C>;
-    default: return pair_sum_kernel<P, T, 4, C>;  // 4 and 8
+    case 4: return pair_sum_kernel<P, T, 4, C>;
+    case 8:
+      // the grid is sized for the R the host asked for, so an R without an
+      // instance must fail rather than fall back to another R
+      if constexpr (C) return pair_sum_kernel<P, T, 8, C>;
+      return nullptr;
+    default: return nullptr;
   }
 }
 
