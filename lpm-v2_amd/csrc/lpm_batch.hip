@@ -125,7 +125,8 @@ __global__ __launch_bounds__(256) void batch_finalize(BatchFinArgs f) {
   f.w[p] = A0 * y - A1 * x;
 }
 
-using BatchFn = void (*)(lpm::BatchArgs);
+using BatchFn = void (*)(const void*, const void*, double*, const lpm::BatchProblem*, const lpm::BatchItem*, int,
+                         int, unsigned int*);
 
 template <typename T, bool USE_E>
 BatchFn pick_batch_r(int r) {
@@ -203,8 +204,10 @@ int plan(lpm_bve_batch* b) {
     p.part_off = part;
     part += (long long)std::max(p.nsplit, 1) * 3 * p.nt;
     const int nw = (p.nt + per_wave - 1) / per_wave;
-    for (int t = 0; t < nw; ++t)
-      for (int s = 0; s < p.nsplit; ++s) {
+    // split-major: the waves that take consecutive items read the same source
+    // chunk, so it stays in L2 while they run
+    for (int s = 0; s < p.nsplit; ++s)
+      for (int t = 0; t < nw; ++t) {
         const int cost = std::min(chunk, nsrc - s * chunk);
         items.push_back({{k, t, s, 0}, cost});
       }
@@ -259,7 +262,7 @@ int eval_velocity(lpm_bve_batch* b, const double* x, const double* y, const doub
   }
   if (b->n_items > 0)
     hipLaunchKernelGGL(pick_batch(b->precision, on_sphere, b->r), dim3((unsigned)b->grid), dim3(256), 0,
-                       ctx->stream, ba);
+                       ctx->stream, ba.tgt, ba.src, ba.partial, ba.probs, ba.items, ba.n_items, ba.kt, ba.counter);
   if (ctx->timing && e0 && e1) {
     LPM_HIP(ctx, hipEventRecord(e1, ctx->stream));
     ctx->pending.push_back({e0, e1, LPM_K_SPHERE_BS});

@@ -560,24 +560,30 @@ struct BatchArgs {
   unsigned int* counter;
 };
 
+// The buffers are separate __restrict__ kernel parameters rather than fields
+// of BatchArgs: the loop stores partials and then reads sources of the next
+// item, and only kernel-level noalias lets the compiler keep the source reads
+// on the scalar unit (wave-uniform s_load, SGPR operands) across those stores.
+// Through a struct, it issued per-lane vector loads of the same addresses.
 template <class P, typename T, int R, bool COMPENSATED>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void pair_batch_kernel(BatchArgs b) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void pair_batch_kernel(
+    const void* __restrict__ tgt, const void* __restrict__ srcv, double* __restrict__ partial,
+    const BatchProblem* __restrict__ probs, const BatchItem* __restrict__ items, int n_items, int kt,
+    unsigned int* __restrict__ counter) {
   const int lane = threadIdx.x & 63;
-  const Vec4<T>* __restrict__ tg = reinterpret_cast<const Vec4<T>*>(b.tgt);
-  const T* __restrict__ src = reinterpret_cast<const T*>(b.src);
+  const Vec4<T>* tg = reinterpret_cast<const Vec4<T>*>(tgt);
+  const T* src = reinterpret_cast<const T*>(srcv);
   for (;;) {
     int item = 0;
-    if (lane == 0) item = (int)atomicAdd(b.counter, 1u);
+    if (lane == 0) item = (int)atomicAdd(counter, 1u);
     item = __builtin_amdgcn_readfirstlane(__shfl(item, 0));
-    if (item >= b.n_items) break;
-    const BatchItem it = b.items[item];
-    const BatchProblem pr = b.probs[it.prob];
+    if (item >= n_items) break;
+    const BatchItem it = items[item];
+    const BatchProblem pr = probs[it.prob];
     const int j_begin = pr.s_off + it.split * pr.chunk;
     const int j_end = min(pr.s_off + pr.ns, j_begin + pr.chunk);
-    pair_tile<P, T, R, COMPENSATED>(tg + pr.t_off, pr.nt, it.wtile * 64 * R, pr.t_off, src, j_begin,
-                                    j_end, b.kt,
-                                    b.partial + pr.part_off + (size_t)it.split * P::NOUT * pr.nt,
-                                    pr.nt);
+    pair_tile<P, T, R, COMPENSATED>(tg + pr.t_off, pr.nt, it.wtile * 64 * R, pr.t_off, src, j_begin, j_end,
+                                    kt, partial + pr.part_off + (size_t)it.split * P::NOUT * pr.nt, pr.nt);
   }
 }
 
