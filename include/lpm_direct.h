@@ -457,8 +457,12 @@ int lpm_kernel_stats(lpm_ctx *ctx, int kind, int64_t *launches, double *total_ms
 int lpm_reset_kernel_stats(lpm_ctx *ctx);
 
 /* Launch shape of a pair kernel: targets per lane (1, 2, 4 or 8), sources
- * per compensated tile (multiple of 8) and source splits (0 = automatic).
- * Values 0 restore the defaults. */
+ * per compensated tile (multiple of 8) and source splits (0 = automatic:
+ * chunks of about 2,048 sources, at least enough splits to fill the GPU,
+ * at most 2 GiB of FP64 split partials).  Values 0 restore the per-kind
+ * defaults.  The FP64 PSE derivative, interpolation and SWE kinds have no
+ * 8-targets-per-lane kernel: their calls then fail with LPM_ERR_ARG.  The
+ * shape changes the schedule, not the result beyond FP32 (FP64) rounding. */
 int lpm_set_tuning(lpm_ctx *ctx, int kind, int precision, int targets_per_lane,
                    int tile_sources, int splits);
 int lpm_get_tuning(lpm_ctx *ctx, int kind, int precision, int *targets_per_lane,
