@@ -441,13 +441,14 @@ int run_pair_sum(lpm_ctx* ctx, const PairProblem& pp) {
   const int nw = ops ? ops_record_words(kind, pp.m) : SREC;
   if (ops && !ov.r) {
     // targets per lane from a sweep of R in {1, 2, 4, 8} on MI355X (512^2
-    // plane, icosahedral L7): the sphere operators (series distance) at R = 1,
-    // the plane derivative sums and interpolation at R = 2, the 7-sum SWE
-    // kernel at R = 1
+    // plane, icosahedral L7): in FP32 the sphere operators (series distance)
+    // at R = 1, the plane derivative sums and interpolation at R = 2, the
+    // 7-sum SWE kernel at R = 1; in FP64 every operator at R = 2 (R = 1 was
+    // 3-7 % slower for the sphere gradient and divergence, R = 4 up to 25 %)
     if (prec == LPM_PREC_FP32_KAHAN)
       tu.r = kind == LPM_K_PSE_DERIV_SPHERE || kind == LPM_K_PSE_INTERP_SPHERE || kind == LPM_K_SWE_PLANE ? 1 : 2;
     else
-      tu.r = no > 4 ? 1 : 2;
+      tu.r = 2;
   }
   const int ns = pp.ns;
   DevBuf& tb = pp.tgt_buf ? *pp.tgt_buf : ctx->tgt;
