@@ -78,14 +78,38 @@ def cpu_baseline(m, zeta, target_seconds: float, max_threads: int):
         if dt > 0.25 * target_seconds or t_total > target_seconds or b >= m.n:
             break
         b = min(m.n, int(b * max(2.0, 0.5 * target_seconds / max(dt, 1e-3))))
-    return {
+    out = {
         "value": rate / 1e9,
         "unit": "G pair-interactions/s",
         "cores": threads,
         "kind": "port",
         "sample": f"FP64 oracle BVESphereVelocity loop, targets [0,{b}) x {ns} active sources "
                   f"({pairs / 1e9:.2f} G pairs, {dt:.1f} s)",
+        "cpu_model": host_cpu_model(),
     }
+    # the same loop on one core, over a block sized to about a tenth of the budget
+    oracle.set_threads(1)
+    b1 = max(16, min(m.n, int(0.1 * target_seconds * rate / threads / max(ns, 1))))
+    t0 = time.perf_counter()
+    oracle.bve_sphere_velocity(m.x, m.y, m.z, zeta, m.area, m.active, 1.0, 0, b1)
+    dt1 = time.perf_counter() - t0
+    pairs1 = b1 * ns - int(m.active[:b1].sum())
+    out["one_core"] = {"value": pairs1 / dt1 / 1e9, "sample": f"targets [0,{b1}) ({pairs1 / 1e9:.2f} G pairs, "
+                                                              f"{dt1:.1f} s)"}
+    oracle.set_threads(max_threads)
+    return out
+
+
+def host_cpu_model() -> str:
+    """The host CPU's model name from /proc/cpuinfo (empty if unreadable)."""
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return ""
 
 
 def pse_ops_cpu_baseline(res: dict, max_threads: int, plane_block: int = 2048, sphere_block: int = 1024):
