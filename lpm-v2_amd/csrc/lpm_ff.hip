@@ -764,7 +764,7 @@ struct MlpCorrArgs {
 // alone takes half a CU's LDS.
 __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(2, 2))) void ff_mlp_kernel(MlpCorrArgs a) {
   __shared__ __attribute__((aligned(16))) __bf16 sfrag[lpm::kMlpFragElems];
-  __shared__ float sbias[lpm::kMlpBiasElems];
+  __shared__ __attribute__((aligned(16))) float sbias[lpm::kMlpBiasElems];
   for (int t = threadIdx.x; t < lpm::kMlpFragElems / 8; t += blockDim.x)
     reinterpret_cast<lpm::bf16x8_t*>(sfrag)[t] = reinterpret_cast<const lpm::bf16x8_t*>(a.frag)[t];
   for (int t = threadIdx.x; t < lpm::kMlpBiasElems; t += blockDim.x) sbias[t] = a.bias[t];

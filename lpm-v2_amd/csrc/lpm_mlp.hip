@@ -119,7 +119,7 @@ __global__ __launch_bounds__(256) void mlp_mfma_forward_kernel(int n, int nin, i
                                                                const __bf16* __restrict__ frag,
                                                                const float* __restrict__ bias, float* __restrict__ Y) {
   __shared__ __attribute__((aligned(16))) __bf16 sfrag[kMlpFragElems];
-  __shared__ float sbias[kMlpBiasElems];
+  __shared__ __attribute__((aligned(16))) float sbias[kMlpBiasElems];
   for (int t = threadIdx.x; t < kMlpFragElems / 8; t += blockDim.x)
     reinterpret_cast<bf16x8_t*>(sfrag)[t] = reinterpret_cast<const bf16x8_t*>(frag)[t];
   for (int t = threadIdx.x; t < kMlpBiasElems; t += blockDim.x) sbias[t] = bias[t];

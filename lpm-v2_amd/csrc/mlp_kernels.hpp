@@ -204,7 +204,8 @@ static __global__ __launch_bounds__(256) void mlp_pack_fragments(const float* __
 // One wavefront evaluates NCT x 16 samples.  in8[ct] holds this lane's 8 input
 // features (features 8 g .. 8 g + 7 of sample 16 ct + l % 16, g = l / 16);
 // on return out[ct][0..2] are the outputs of that sample in lanes 0..15
-// (lanes 16..63 hold padding rows).  Weights/biases in LDS (fragment order).
+// (lanes 16..63 hold padding rows).  Weights/biases in LDS (fragment order);
+// sbias must be 16-byte aligned (it is read as float4 rows).
 template <int NCT, int ACT = kActTanh>
 __device__ __forceinline__ void mlp_wave_forward(const bf16x8_t in8[NCT], const __bf16* __restrict__ sfrag,
                                                  const float* __restrict__ sbias, float out[NCT][4]) {
@@ -217,18 +218,22 @@ __device__ __forceinline__ void mlp_wave_forward(const bf16x8_t in8[NCT], const 
   floatx4_t acc[8][NCT];
   bf16x8_t h[4][NCT];  // B operands of the next layer: [ks][ct]
 
+  // The accumulators start at the bias (lane group g holds rows 4 g .. 4 g + 3
+  // of each 16-row tile), so the MFMA adds it and the activation pass needs no
+  // separate add: a fifth of the vector instructions of a forward pass.
+  auto bias4 = [&](const float* b, int rt) -> floatx4_t {
+    return *reinterpret_cast<const floatx4_t*>(b + 16 * rt + 4 * g);
+  };
   // layer 1: K = 32 (one step)
 #pragma unroll
   for (int rt = 0; rt < 8; ++rt) {
     const bf16x8_t a = F1[rt * 64 + lane];
+    const floatx4_t z = bias4(sbias, rt);
 #pragma unroll
-    for (int ct = 0; ct < NCT; ++ct) {
-      floatx4_t z = {0.f, 0.f, 0.f, 0.f};
-      acc[rt][ct] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, in8[ct], z, 0, 0, 0);
-    }
+    for (int ct = 0; ct < NCT; ++ct) acc[rt][ct] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, in8[ct], z, 0, 0, 0);
   }
-  // bias + tanh -> next B operands
-  auto activate = [&](const float* b) {
+  // activation -> next B operands
+  auto activate = [&]() {
 #pragma unroll
     for (int ks = 0; ks < 4; ++ks)
 #pragma unroll
@@ -237,21 +242,22 @@ __device__ __forceinline__ void mlp_wave_forward(const bf16x8_t in8[NCT], const 
 #pragma unroll
         for (int e = 0; e < 8; ++e) {
           const int rt = 2 * ks + e / 4, vv = e % 4;
-          const float z = acc[rt][ct][vv] + b[16 * rt + 4 * g + vv];
-          v[e] = (__bf16)act_fast<ACT>(z);
+          v[e] = (__bf16)act_fast<ACT>(acc[rt][ct][vv]);
         }
         h[ks][ct] = v;
       }
   };
-  activate(sbias);
+  activate();
   // layers 2, 3: 128 x 128
 #pragma unroll 1
   for (int layer = 0; layer < 2; ++layer) {
     const bf16x8_t* F = layer == 0 ? F2 : F3;
+    const float* bl = sbias + 128 * (layer + 1);
 #pragma unroll
     for (int rt = 0; rt < 8; ++rt) {
+      const floatx4_t z = bias4(bl, rt);
 #pragma unroll
-      for (int ct = 0; ct < NCT; ++ct) acc[rt][ct] = (floatx4_t){0.f, 0.f, 0.f, 0.f};
+      for (int ct = 0; ct < NCT; ++ct) acc[rt][ct] = z;
 #pragma unroll
       for (int ks = 0; ks < 4; ++ks) {
         const bf16x8_t a = F[(rt * 4 + ks) * 64 + lane];
@@ -260,12 +266,13 @@ __device__ __forceinline__ void mlp_wave_forward(const bf16x8_t in8[NCT], const 
           acc[rt][ct] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, h[ks][ct], acc[rt][ct], 0, 0, 0);
       }
     }
-    activate(sbias + 128 * (layer + 1));
+    activate();
   }
   // output layer: one 16-row tile, K = 128
   floatx4_t o[NCT];
+  const floatx4_t bo = bias4(sbias + 384, 0);
 #pragma unroll
-  for (int ct = 0; ct < NCT; ++ct) o[ct] = (floatx4_t){0.f, 0.f, 0.f, 0.f};
+  for (int ct = 0; ct < NCT; ++ct) o[ct] = bo;
 #pragma unroll
   for (int ks = 0; ks < 4; ++ks) {
     const bf16x8_t a = F4[ks * 64 + lane];
@@ -275,7 +282,7 @@ __device__ __forceinline__ void mlp_wave_forward(const bf16x8_t in8[NCT], const 
 #pragma unroll
   for (int ct = 0; ct < NCT; ++ct)
 #pragma unroll
-    for (int v = 0; v < 4; ++v) out[ct][v] = o[ct][v] + sbias[384 + 4 * g + v];
+    for (int v = 0; v < 4; ++v) out[ct][v] = o[ct][v];
 }
 
 }  // namespace lpm
