@@ -365,8 +365,13 @@ int blocks_per_cu(const void* kfn) {
 // kChunkTarget sources, and at least enough splits that the blocks fill the
 // chip in whole rounds of resident blocks (round efficiency B / (rounds *
 // slots) >= 0.9, for small problems), within max_split_mem splits (the FP64
-// partials of all splits are kept until the ordered reduction).
+// partials of all splits are kept until the ordered reduction).  Problems
+// with few target tiles (2*10^5 targets: 192 blocks per split) also want
+// kMinRounds rounds of blocks, down to chunks of kChunkMin sources (cubed
+// sphere L7: 4.33 ms at 2,048 sources per chunk, 4.23 ms at 1,024).
 constexpr int kChunkTarget = 2048;
+constexpr int kChunkMin = 1024;
+constexpr int kMinRounds = 12;
 constexpr size_t kPartialCap = size_t(2) << 30;  // bytes of FP64 split partials
 
 static int choose_splits(int n_wtiles, int ns, int kt, int slots, int max_split_mem) {
@@ -390,7 +395,9 @@ static int choose_splits(int n_wtiles, int ns, int kt, int slots, int max_split_
     }
   }
   const int by_chunk = (ns + kChunkTarget - 1) / kChunkTarget;
-  return std::max(1, std::min(max_split, std::max(fill, by_chunk)));
+  const long long by_rounds = ((long long)kMinRounds * slots + nblk_t - 1) / std::max(nblk_t, 1);
+  const int rounds_split = (int)std::min<long long>(by_rounds, std::max(1, ns / kChunkMin));
+  return std::max(1, std::min(max_split, std::max(fill, std::max(by_chunk, rounds_split))));
 }
 
 hipEvent_t take_event(lpm_ctx* ctx) {
