@@ -15,6 +15,9 @@ pytestmark = pytest.mark.gpu
 SEEDS = [11, 12, 13, 14, 15, 16]
 SIZES = [1, 2, 63, 65, 257, 1000, 2500]
 FP32_TOL = 5e-4
+# FP32 PSE and SWE sums: outputs are differences of larger terms (the full-size
+# tests hold them to 1e-4 on meshes); random points and small n add spread
+PSE32_TOL = 2e-3
 
 
 def _separated(p, dmin):
@@ -116,3 +119,39 @@ def test_ragged_sums_match_oracle(ctx, oracle, seed):
         ref = oracle.pse_plane_interpolate(px, py, g, parea, pact, pe, qx, qy)
         _close(ctx.pse_plane_interpolate(px, py, g, parea, pact, pe, qx, qy, precision="fp64"), ref, 1e-11,
                "plane interpolate fp64 " + tag)
+        _close(ctx.pse_plane_interpolate(px, py, g, parea, pact, pe, qx, qy, precision="fp32"), ref, PSE32_TOL,
+               "plane interpolate fp32 " + tag)
+
+
+@pytest.mark.parametrize("seed", SEEDS[:3])
+def test_ragged_fp32_pse_sums_match_oracle(ctx, oracle, seed):
+    """The FP32 PSE, derivative and SWE kernels on ragged inputs: their tails
+    run at 1 or 2 targets per lane, other shapes than the FP64 kernels."""
+    rng = np.random.default_rng(100 + seed)
+    for n_req in SIZES:
+        x, y, z, area, act = _sphere(rng, n_req)
+        n = x.size
+        i0, i1 = _block(rng, n)
+        tag = f"seed {seed} n {n} block [{i0}, {i1}) active {int(act.sum())}"
+        eps = 0.5 / max(n, 4) ** 0.25
+        f = np.sin(2 * z) + x
+        ref = oracle.pse_sphere_laplacian(x, y, z, f, area, act, eps, 1.0, i0, i1)
+        _close(ctx.pse_sphere_laplacian(x, y, z, f, area, act, eps, i0=i0, i1=i1, precision="fp32"), ref,
+               PSE32_TOL, "sphere pse fp32 " + tag)
+        ref = oracle.pse_sphere_gradient(x, y, z, f, area, act, eps, 1.0, i0, i1)
+        _close(ctx.pse_sphere_gradient(x, y, z, f, area, act, eps, i0=i0, i1=i1, precision="fp32"), ref,
+               PSE32_TOL, "sphere gradient fp32 " + tag)
+        px, py, parea, pact = _plane(rng, n)
+        q = rng.normal(size=n)
+        pe = 2.0 / max(n, 4) ** 0.5
+        g = np.exp(-2 * (px * px + py * py))
+        ref = oracle.pse_plane_laplacian(px, py, g, parea, pact, pe, 8, i0, i1)
+        _close(ctx.pse_plane_laplacian(px, py, g, parea, pact, pe, order=8, i0=i0, i1=i1, precision="fp32"), ref,
+               PSE32_TOL, "plane pse8 fp32 " + tag)
+        ref = oracle.pse_plane_gradient(px, py, g, parea, pact, pe, i0, i1)
+        _close(ctx.pse_plane_gradient(px, py, g, parea, pact, pe, i0=i0, i1=i1, precision="fp32"), ref,
+               PSE32_TOL, "plane gradient fp32 " + tag)
+        hh = 1.0 + 0.1 * px
+        ref = oracle.swe_plane_rhs(px, py, q, 0.3 * q, hh, 0.05 * py, parea, pact, pe, i0, i1)
+        _close(ctx.swe_plane_rhs(px, py, q, 0.3 * q, hh, 0.05 * py, parea, pact, pe, i0=i0, i1=i1,
+                                 precision="fp32"), ref, PSE32_TOL, "swe fp32 " + tag)
