@@ -112,6 +112,77 @@ def host_cpu_model() -> str:
     return ""
 
 
+def cfg_cpu_baseline(workload: str, res: dict, target_seconds: float, max_threads: int):
+    """The FP64 oracle (the reference loops restated in C, OpenMP over contiguous
+    target blocks) beside a cfg workload, on the box's host cores.  cfg0 runs
+    whole (its 10 RK4 steps); cfg1 and cfg3 time the sphere sum on a bounded
+    target block and extrapolate a step (4 sums); cfg2 times the PSE sums on a
+    bounded block and extrapolates one evaluation.  Adds res["cpu_baseline"]."""
+    import types
+
+    from lpmgpu import workloads
+    from oracle import oracle
+
+    threads = oracle.set_threads(max_threads)
+    if workload == "cfg0":
+        w = workloads.cfg0()
+        m = w["mesh"]
+        u, v, z = oracle.bve_sphere_velocity(m.x, m.y, m.z, w["relvort"], m.area, m.active)
+        st = dict(x=m.x, y=m.y, z=m.z, relvort=w["relvort"], u=u, v=v, w=z)
+        t0 = time.perf_counter()
+        oracle.bve_timestep(st, m.area, m.active, 1.0, w["omega"], w["dt"], nsteps=w["steps"])
+        ms = 1e3 * (time.perf_counter() - t0) / w["steps"]
+        out = {"ms_per_step": round(ms, 3), "unit": "ms per RK4 step", "cores": threads, "kind": "port",
+               "sample": f"the whole workload: {w['steps']} FP64 RK4 steps (4 sums each) at icosahedral L4"}
+        if res.get("ms_per_step"):
+            out["gpu_speedup"] = round(ms / res["ms_per_step"], 1)
+    elif workload in ("cfg1", "cfg3"):
+        if workload == "cfg1":
+            w = workloads.cfg1()
+            m, zeta = w["mesh"], w["relvort"]
+            pairs_step = 4 * (m.n * m.n_active - m.n_active)
+        else:
+            probs = workloads.cfg3()["problems"]
+            p = max(probs, key=lambda q: q["x"].shape[0])
+            m = types.SimpleNamespace(x=p["x"], y=p["y"], z=p["z"], area=p["area"], active=p["active"],
+                                      n=p["x"].shape[0])
+            zeta = p["relvort"]
+            pairs_step = 4 * res["pairs_per_eval"]
+        out = cpu_baseline(m, zeta, target_seconds, max_threads)
+        ms = pairs_step / (out["value"] * 1e9) * 1e3
+        out["ms_per_step_extrapolated"] = round(ms, 1)
+        if res.get("ms_per_step"):
+            out["gpu_speedup"] = round(ms / res["ms_per_step"], 1)
+    elif workload == "cfg2":
+        w = workloads.cfg2(512)
+        x, y, area, eps, f = w["x"], w["y"], w["area"], w["eps"], w["f"]
+        n = x.shape[0]
+        act = np.ones(n, np.int32)
+        out = {"unit": "G pair-interactions/s", "cores": threads, "kind": "port"}
+        for order in (2, 8):
+            b = 256
+            while True:
+                t0 = time.perf_counter()
+                oracle.pse_plane_laplacian(x, y, f, area, act, eps, order, 0, b)
+                dt = time.perf_counter() - t0
+                if dt > 0.2 * target_seconds or b >= n:
+                    break
+                b = min(n, int(b * max(2.0, 0.4 * target_seconds / max(dt, 1e-3))))
+            rate = b * n / dt
+            entry = {"value": rate / 1e9, "sample": f"targets [0,{b}) x {n} sources ({b * n / 1e9:.2f} G pairs, "
+                                                    f"{dt:.1f} s)",
+                     "ms_per_evaluation_extrapolated": round(n * n / rate * 1e3, 1)}
+            k = (res.get(f"pse{order}") or {}).get("pair_kernel") or {}
+            if k.get("avg_ms"):
+                entry["gpu_speedup"] = round(entry["ms_per_evaluation_extrapolated"] / k["avg_ms"], 1)
+            out[f"pse{order}"] = entry
+    else:
+        return
+    oracle.set_threads(max_threads)
+    res["cpu_baseline"] = out
+    log(f"[cpu {workload}] {out}")
+
+
 def pse_ops_cpu_baseline(res: dict, max_threads: int, plane_block: int = 2048, sphere_block: int = 1024):
     """FP64 oracle (the reference loops restated in C, OpenMP over a contiguous
     target block) for each operator of the pse_ops workload, on the same inputs
@@ -286,6 +357,8 @@ def main():
         res = fn(ctx, log=log, **kw)
         if args.workload == "pse_ops" and args.cpu_seconds > 0:
             pse_ops_cpu_baseline(res, args.cpu_threads)
+        elif args.cpu_seconds > 0:
+            cfg_cpu_baseline(args.workload, res, args.cpu_seconds, args.cpu_threads)
         res["metric"] = METRIC
         print(json.dumps(res), flush=True)
         ctx.close()
