@@ -279,7 +279,8 @@ static KernelFn pick_kernel(int kind, int prec, int r, bool variant) {
     case LPM_K_PLANE_BS: return pick_r<PlaneBS<double>, double>(r);
     case LPM_K_PSE8: return pick_r<PsePolicy<double, 8>, double>(r);
     case LPM_K_PSE2: return pick_r<PsePolicy<double, 2>, double>(r);
-    case LPM_K_PSE_SPHERE8: return pick_r<PseSphere<double, false>, double>(r);
+    case LPM_K_PSE_SPHERE8:
+      return variant ? pick_r<PseSphere<double, true>, double>(r) : pick_r<PseSphere<double, false>, double>(r);
     case LPM_K_SPHERE_GREEN:
       return on_sphere ? pick_r<SphereGreen<double, false>, double>(r) : pick_r<SphereGreen<double, true>, double>(r);
     default: return pick_r<PlaneGreen<double>, double>(r);
@@ -464,10 +465,12 @@ int run_pair_sum(lpm_ctx* ctx, const PairProblem& pp) {
 
   const int per_wave = 64 * tu.r;
   const int n_wtiles = (nt + per_wave - 1) / per_wave;
-  // sphere PSE kinds in FP32: the series form of the distance when 7 eps <= 0.56 R
+  // sphere PSE kinds: the series form of the distance when 7 eps <= 0.56 R
+  // (FP32, 7 terms) or 7 eps <= 0.35 R (FP64, 10 terms)
   const bool sphere_pse = kind == LPM_K_PSE_SPHERE8 || kind == LPM_K_PSE_DERIV_SPHERE ||
                           kind == LPM_K_PSE_INTERP_SPHERE;
-  const bool variant = sphere_pse ? (7.0 * pp.eps <= 0.56 * pp.radius) : pp.on_sphere;
+  const double series_reach = prec == LPM_PREC_FP32_KAHAN ? 0.56 : 0.35;
+  const bool variant = sphere_pse ? (7.0 * pp.eps <= series_reach * pp.radius) : pp.on_sphere;
   KernelFn fn = ops ? reinterpret_cast<KernelFn>(const_cast<void*>(ops_pick_kernel(kind, prec, tu.r, pp.m, variant)))
                     : pick_kernel(kind, prec, tu.r, variant);
   if (!fn) return fail(ctx, LPM_ERR_ARG, "no pair kernel for this kind, precision and field count");

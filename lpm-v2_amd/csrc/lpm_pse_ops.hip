@@ -206,28 +206,23 @@ KernelFn pick_r(int r) {
 
 template <typename T>
 KernelFn pick_t(int kind, int r, int m, bool series) {
-  constexpr bool F = sizeof(T) == 4;
   switch (kind) {
     case LPM_K_PSE_DERIV_PLANE:
       if (m == 1) return pick_r<PseDeriv<T, false, 1, false>, T>(r);
       if (m == 2) return pick_r<PseDeriv<T, false, 2, false>, T>(r);
       return nullptr;
     case LPM_K_PSE_DERIV_SPHERE:
-      if constexpr (F) {
-        if (series) {
-          if (m == 1) return pick_r<PseDeriv<T, true, 1, true>, T>(r);
-          if (m == 3) return pick_r<PseDeriv<T, true, 3, true>, T>(r);
-          return nullptr;
-        }
+      if (series) {
+        if (m == 1) return pick_r<PseDeriv<T, true, 1, true>, T>(r);
+        if (m == 3) return pick_r<PseDeriv<T, true, 3, true>, T>(r);
+        return nullptr;
       }
       if (m == 1) return pick_r<PseDeriv<T, true, 1, false>, T>(r);
       if (m == 3) return pick_r<PseDeriv<T, true, 3, false>, T>(r);
       return nullptr;
     case LPM_K_PSE_INTERP_PLANE: return pick_r<PseInterp<T, false, false>, T>(r);
     case LPM_K_PSE_INTERP_SPHERE:
-      if constexpr (F) {
-        if (series) return pick_r<PseInterp<T, true, true>, T>(r);
-      }
+      if (series) return pick_r<PseInterp<T, true, true>, T>(r);
       return pick_r<PseInterp<T, true, false>, T>(r);
     case LPM_K_SWE_PLANE: return pick_r<SwePlane<T>, T>(r);
     default: return nullptr;
@@ -275,7 +270,7 @@ int ops_record_words(int kind, int m) {
 }
 
 const void* ops_pick_kernel(int kind, int prec, int r, int m, bool series) {
-  KernelFn fn = prec == LPM_PREC_FP32_KAHAN ? pick_t<float>(kind, r, m, series) : pick_t<double>(kind, r, m, false);
+  KernelFn fn = prec == LPM_PREC_FP32_KAHAN ? pick_t<float>(kind, r, m, series) : pick_t<double>(kind, r, m, series);
   return reinterpret_cast<const void*>(fn);
 }
 

@@ -210,8 +210,10 @@ struct PsePolicy {
 // u = |x'_i - x'_j|^2 / (k R)^2 = (c / R)^2 (c the chord) the scaled squared
 // distance is q = (k R)^2 theta^2, theta = 2 asin(sqrt(u) / 2).
 // SERIES: theta^2 = u g(u), g = sum_n a_n u^(n-1), a_n = 2 / (n^2 binom(2n, n))
-// (the Taylor series of 4 asin^2(s / 2)), through n = 7; the caller uses it
-// when 7 eps <= 0.56 R, where the first dropped term is below 4e-9 relative.
+// (the Taylor series of 4 asin^2(s / 2)), through n = 7 in FP32 (the caller
+// uses it when 7 eps <= 0.56 R, where the first dropped term is below 4e-9
+// relative) and n = 10 in FP64 (7 eps <= 0.35 R).  Pairs farther than 7 eps
+// get a slightly low theta^2, but their kernel factor exp(-q) is below e^-49.
 // Source record: {x', y', z', f A, A, idx, 1/(kR)^2, (kR)^2}; target {x', y', z', f}.
 // Scaled squared great-circle distance q = (k R theta)^2 from the squared
 // chord c2 = |x'_i - x'_j|^2 of directions scaled by k R; inv = 1/(k R)^2,
@@ -219,7 +221,20 @@ struct PsePolicy {
 template <typename T, bool SERIES>
 __device__ __forceinline__ T sphere_q(T c2, T inv, T kr2) {
   const T u = c2 * inv;
-  if constexpr (SERIES) {
+  if constexpr (SERIES && sizeof(T) == 8) {
+    // FP64: through n = 10; the caller uses it when 7 eps <= 0.35 R, where the
+    // first dropped term is below 2e-17 relative
+    double g = fma(u, 1.0 / 9237800.0, 1.0 / 1969110.0);
+    g = fma(u, g, 1.0 / 411840.0);
+    g = fma(u, g, 1.0 / 84084.0);
+    g = fma(u, g, 1.0 / 16632.0);
+    g = fma(u, g, 1.0 / 3150.0);
+    g = fma(u, g, 1.0 / 560.0);
+    g = fma(u, g, 1.0 / 90.0);
+    g = fma(u, g, 1.0 / 12.0);
+    g = fma(u, g, 1.0);
+    return c2 * g;
+  } else if constexpr (SERIES) {
     T g = fma(u, (T)(1.0 / 84084.0), (T)(1.0 / 16632.0));
     g = fma(u, g, (T)(1.0 / 3150.0));
     g = fma(u, g, (T)(1.0 / 560.0));

@@ -87,6 +87,37 @@ def test_sphere_gradient(ctx, oracle, precision, level, power):
     assert e < TOL[precision]
 
 
+def test_fp64_series_distance_matches_oracle(ctx, oracle):
+    """eps = 0.045 at nest 5 gives 7 eps <= 0.35 R: the FP64 kernels take the
+    10-term series of the great-circle distance, the oracle asin."""
+    m = icos(5)
+    f = harmonic54(m.x, m.y, m.z)
+    eps = 0.045
+    u, v, w = harmonic54_gradient(m.x, m.y, m.z)
+    i1 = 5000
+    cases = {
+        "laplacian": (oracle.pse_sphere_laplacian(m.x, m.y, m.z, f, m.area, m.active, eps, 1.0, 0, i1),
+                      ctx.pse_sphere_laplacian(m.x, m.y, m.z, f, m.area, m.active, eps, i1=i1, precision="fp64")),
+        "gradient": (oracle.pse_sphere_gradient(m.x, m.y, m.z, f, m.area, m.active, eps, i1=i1),
+                     ctx.pse_sphere_gradient(m.x, m.y, m.z, f, m.area, m.active, eps, i1=i1, precision="fp64")),
+        "divergence": (oracle.pse_sphere_divergence(m.x, m.y, m.z, u, v, w, m.area, m.active, eps, i1=i1),
+                       ctx.pse_sphere_divergence(m.x, m.y, m.z, u, v, w, m.area, m.active, eps, i1=i1,
+                                                 precision="fp64")),
+    }
+    qx, qy, qz = latlon_grid()
+    sel = slice(0, None, 7)
+    cases["interpolation"] = (
+        oracle.pse_sphere_interpolate(m.x, m.y, m.z, f, m.area, m.active, eps, qx[sel], qy[sel], qz[sel]),
+        ctx.pse_sphere_interpolate(m.x, m.y, m.z, f, m.area, m.active, eps, qx[sel], qy[sel], qz[sel],
+                                   precision="fp64"))
+    for name, (ref, got) in cases.items():
+        ref = np.atleast_2d(np.asarray(ref))[:, :i1] if name != "interpolation" else ref
+        got = np.atleast_2d(np.asarray(got))[:, :i1] if name != "interpolation" else got
+        e = _rel(got, ref)
+        print("fp64 series", name, e)
+        assert e < 1e-11, (name, e)
+
+
 @pytest.mark.parametrize("precision", ["fp64", "fp32"])
 def test_sphere_double_dot_and_divergence(ctx, oracle, precision):
     m, f, eps = _sphere(4)
