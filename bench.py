@@ -342,7 +342,7 @@ def main():
         return
 
     if args.gpus != 1:
-        print(json.dumps({"error": "this build measures one GPU; the direct sum would shard by target blocks "
+        print(json.dumps({"error": "bench.py measures one GPU; the direct sum would shard by target blocks "
                           "(each GPU all sources, a slice of the targets), --gpus N > 1 is not run"}))
         sys.exit(2)
 
