@@ -80,3 +80,19 @@ def test_batch_autotune_and_tile_shapes_agree(ctx):
         b2.delete()
     b.delete()
     assert np.isfinite(ref["u"]).all()
+
+
+def test_batch_tuning_rejects_bad_shapes(ctx):
+    """Shapes without a kernel instance or with a chunk shorter than a tile
+    are refused with an error, and the batch keeps its previous shape."""
+    from lpmgpu.api import LpmError
+
+    b = BVEBatch(ctx, _small(3, seed=11), precision="fp32")
+    before = b.tuning()
+    for shape in [(1, 256, 16384), (3, 256, 16384), (4, 100, 16384), (4, 0, 16384), (4, 256, 128)]:
+        with pytest.raises(LpmError):
+            b.set_tuning(*shape)
+        assert b.tuning() == before
+    b.timestep(0.01)
+    assert np.isfinite(b.state()["u"]).all()
+    b.delete()
