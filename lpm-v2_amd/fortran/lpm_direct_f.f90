@@ -277,6 +277,247 @@ module lpm_direct_f
       import :: c_int, c_ptr
       type(c_ptr), value :: solver
     end function
+
+    integer(c_int) function lpm_bve_solver_steps(solver, dt, nsteps) bind(C, name="lpm_bve_solver_steps")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: solver
+      real(c_double), value :: dt
+      integer(c_int), value :: nsteps
+    end function
+
+    integer(c_int) function lpm_bve_solver_diagnostics(solver, ke, enstrophy, circulation) &
+        bind(C, name="lpm_bve_solver_diagnostics")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: solver
+      real(c_double), intent(out) :: ke, enstrophy, circulation
+    end function
+
+    ! ------------------------------------------------------------- helpers
+    ! PSE%eps = max_edge_length ** power (PSEDirectSum.f90:93-113)
+    integer(c_int) function lpm_pse_eps(max_edge_length, power, eps) bind(C, name="lpm_pse_eps")
+      import :: c_int, c_double
+      real(c_double), value :: max_edge_length, power
+      real(c_double), intent(out) :: eps
+    end function
+
+    ! MPISetup LoadBalance (MPISetup.f90:132-146): 1-based inclusive blocks, arrays of n_procs
+    integer(c_int) function lpm_load_balance(n_items, n_procs, index_start, index_end, message_length) &
+        bind(C, name="lpm_load_balance")
+      import :: c_int
+      integer(c_int), value :: n_items, n_procs
+      integer(c_int) :: index_start(*), index_end(*), message_length(*)
+    end function
+
+    ! TotalKE / TotalEnstrophy / circulation (SphereBVE.f90:410-441); w may be c_null_ptr (plane)
+    integer(c_int) function lpm_diagnostics(ctx, u, v, w, vort, area, active, n, ke, enstrophy, circulation) &
+        bind(C, name="lpm_diagnostics")
+      import :: c_int, c_ptr, c_double, c_int32_t
+      type(c_ptr), value :: ctx, w
+      real(c_double), intent(in) :: u(*), v(*), vort(*), area(*)
+      integer(c_int32_t), intent(in) :: active(*)
+      integer(c_int), value :: n
+      real(c_double), intent(out) :: ke, enstrophy, circulation
+    end function
+
+    ! ------------------------------------------------ planar solver (PlaneSolver)
+    ! New / Timestep / Delete (PlaneIncompressibleSolver.f90:66-76, 103-259); u, v may be c_null_ptr
+    integer(c_int) function lpm_plane_solver_new(ctx, solver, n, x, y, vort, area, active, u, v, precision, &
+        flags) bind(C, name="lpm_plane_solver_new")
+      import :: c_int, c_ptr, c_double, c_int32_t
+      type(c_ptr), value :: ctx
+      type(c_ptr), intent(out) :: solver
+      integer(c_int), value :: n, precision, flags
+      real(c_double), intent(in) :: x(*), y(*), vort(*), area(*)
+      integer(c_int32_t), intent(in) :: active(*)
+      type(c_ptr), value :: u, v
+    end function
+
+    integer(c_int) function lpm_plane_solver_timestep(solver, dt) bind(C, name="lpm_plane_solver_timestep")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: solver
+      real(c_double), value :: dt
+    end function
+
+    integer(c_int) function lpm_plane_solver_steps(solver, dt, nsteps) bind(C, name="lpm_plane_solver_steps")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: solver
+      real(c_double), value :: dt
+      integer(c_int), value :: nsteps
+    end function
+
+    integer(c_int) function lpm_plane_solver_get_state(solver, x, y, u, v) bind(C, name="lpm_plane_solver_get_state")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: solver
+      real(c_double) :: x(*), y(*), u(*), v(*)
+    end function
+
+    integer(c_int) function lpm_plane_solver_stream(solver, psi) bind(C, name="lpm_plane_solver_stream")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: solver
+      real(c_double) :: psi(*)
+    end function
+
+    integer(c_int) function lpm_plane_solver_diagnostics(solver, ke, enstrophy, circulation) &
+        bind(C, name="lpm_plane_solver_diagnostics")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: solver
+      real(c_double), intent(out) :: ke, enstrophy, circulation
+    end function
+
+    integer(c_int) function lpm_plane_solver_delete(solver) bind(C, name="lpm_plane_solver_delete")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: solver
+    end function
+
+    ! --------------------------------- batched sphere problems (no reference counterpart)
+    ! problem k owns particles offsets(k+1)+1 .. offsets(k+2) (0-based offsets, n_inst + 1 of them)
+    integer(c_int) function lpm_bve_batch_new(ctx, batch, n_inst, offsets, x, y, z, relvort, area, active, &
+        radius, rotation_rate, precision, flags) bind(C, name="lpm_bve_batch_new")
+      import :: c_int, c_ptr, c_double, c_int32_t, c_int64_t
+      type(c_ptr), value :: ctx
+      type(c_ptr), intent(out) :: batch
+      integer(c_int), value :: n_inst, precision, flags
+      integer(c_int64_t), intent(in) :: offsets(*)
+      real(c_double), intent(in) :: x(*), y(*), z(*), relvort(*), area(*), radius(*), rotation_rate(*)
+      integer(c_int32_t), intent(in) :: active(*)
+    end function
+
+    integer(c_int) function lpm_bve_batch_steps(batch, dt, nsteps) bind(C, name="lpm_bve_batch_steps")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: batch
+      real(c_double), value :: dt
+      integer(c_int), value :: nsteps
+    end function
+
+    integer(c_int) function lpm_bve_batch_get_state(batch, x, y, z, relvort, u, v, w) &
+        bind(C, name="lpm_bve_batch_get_state")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: batch
+      real(c_double) :: x(*), y(*), z(*), relvort(*), u(*), v(*), w(*)
+    end function
+
+    integer(c_int) function lpm_bve_batch_autotune(batch, targets_per_lane, tile_sources, chunk_sources, &
+        best_ms) bind(C, name="lpm_bve_batch_autotune")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: batch
+      integer(c_int), intent(out) :: targets_per_lane, tile_sources, chunk_sources
+      real(c_double), intent(out) :: best_ms
+    end function
+
+    integer(c_int) function lpm_bve_batch_delete(batch) bind(C, name="lpm_bve_batch_delete")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: batch
+    end function
+
+    ! ------------------------------------- learned far field (no reference counterpart)
+    ! cell(i) in [0, ncell) is particle i's coarse cell at set-up; flags: expansion order (0 = 4)
+    integer(c_int) function lpm_bve_ff_new(ctx, ff, n, x, y, z, relvort, area, active, cell, ncell, radius, &
+        rotation_rate, near_radius, band_radius, flags) bind(C, name="lpm_bve_ff_new")
+      import :: c_int, c_ptr, c_double, c_int32_t
+      type(c_ptr), value :: ctx
+      type(c_ptr), intent(out) :: ff
+      integer(c_int), value :: n, ncell, flags
+      real(c_double), intent(in) :: x(*), y(*), z(*), relvort(*), area(*)
+      integer(c_int32_t), intent(in) :: active(*), cell(*)
+      real(c_double), value :: radius, rotation_rate, near_radius, band_radius
+    end function
+
+    ! mode: 0 plain direct sum, 1 near + expansion, 2 near + expansion + network
+    integer(c_int) function lpm_bve_ff_velocity(ff, mode, u, v, w) bind(C, name="lpm_bve_ff_velocity")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ff
+      integer(c_int), value :: mode
+      real(c_double) :: u(*), v(*), w(*)
+    end function
+
+    integer(c_int) function lpm_bve_ff_collect(ff, n_samples, seed, total) bind(C, name="lpm_bve_ff_collect")
+      import :: c_int, c_ptr, c_int64_t
+      type(c_ptr), value :: ff
+      integer(c_int), value :: n_samples
+      integer(c_int64_t), value :: seed
+      integer(c_int64_t), intent(out) :: total
+    end function
+
+    integer(c_int) function lpm_bve_ff_train(ff, epochs, batch, lr, seed, final_loss) bind(C, name="lpm_bve_ff_train")
+      import :: c_int, c_ptr, c_double, c_int64_t
+      type(c_ptr), value :: ff
+      integer(c_int), value :: epochs, batch
+      real(c_double), value :: lr
+      integer(c_int64_t), value :: seed
+      real(c_double), intent(out) :: final_loss
+    end function
+
+    integer(c_int) function lpm_bve_ff_set_online(ff, n_samples, epochs, batch, lr) &
+        bind(C, name="lpm_bve_ff_set_online")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ff
+      integer(c_int), value :: n_samples, epochs, batch
+      real(c_double), value :: lr
+    end function
+
+    integer(c_int) function lpm_bve_ff_steps(ff, dt, nsteps, mode, judge_every, tol, mode_out, last_error, &
+        judged) bind(C, name="lpm_bve_ff_steps")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ff
+      real(c_double), value :: dt, tol
+      integer(c_int), value :: nsteps, mode, judge_every
+      integer(c_int), intent(out) :: mode_out, judged
+      real(c_double), intent(out) :: last_error
+    end function
+
+    integer(c_int) function lpm_bve_ff_get_state(ff, x, y, z, relvort, u, v, w) bind(C, name="lpm_bve_ff_get_state")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ff
+      real(c_double) :: x(*), y(*), z(*), relvort(*), u(*), v(*), w(*)
+    end function
+
+    ! replace positions and relative vorticity (the plain velocity of the new state is stored)
+    integer(c_int) function lpm_bve_ff_set_state(ff, x, y, z, relvort) bind(C, name="lpm_bve_ff_set_state")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ff
+      real(c_double), intent(in) :: x(*), y(*), z(*), relvort(*)
+    end function
+
+    integer(c_int) function lpm_bve_ff_get_params(ff, params, count) bind(C, name="lpm_bve_ff_get_params")
+      import :: c_int, c_ptr, c_float, c_int64_t
+      type(c_ptr), value :: ff
+      real(c_float) :: params(*)
+      integer(c_int64_t), value :: count
+    end function
+
+    integer(c_int) function lpm_bve_ff_set_params(ff, params, count) bind(C, name="lpm_bve_ff_set_params")
+      import :: c_int, c_ptr, c_float, c_int64_t
+      type(c_ptr), value :: ff
+      real(c_float), intent(in) :: params(*)
+      integer(c_int64_t), value :: count
+    end function
+
+    integer(c_int) function lpm_bve_ff_delete(ff) bind(C, name="lpm_bve_ff_delete")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ff
+    end function
+
+    ! ------------------------------------------------------- timing / tuning
+    integer(c_int) function lpm_set_timing(ctx, enable) bind(C, name="lpm_set_timing")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: enable
+    end function
+
+    integer(c_int) function lpm_kernel_stats(ctx, kind, launches, total_ms) bind(C, name="lpm_kernel_stats")
+      import :: c_int, c_ptr, c_double, c_int64_t
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: kind
+      integer(c_int64_t), intent(out) :: launches
+      real(c_double), intent(out) :: total_ms
+    end function
+
+    integer(c_int) function lpm_set_tuning(ctx, kind, precision, targets_per_lane, tile_sources, splits) &
+        bind(C, name="lpm_set_tuning")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: kind, precision, targets_per_lane, tile_sources, splits
+    end function
   end interface
 
 contains

@@ -1,6 +1,7 @@
 !> @file lpm_fortran_smoke.f90
 !> Fortran ISO_C_BINDING smoke program: reads particles from a raw binary file
-!> (int32 n, then x, y, z, relVort, area as float64[n], then int32 active[n]),
+!> (int32 n, then x, y, z, relVort, area as float64[n], int32 active[n], then
+!> int32 ncell and int32 cell[n], each particle's coarse cell),
 !> calls lpm_BVESphereVelocity over two MPISetup-style blocks plus one RK4
 !> step of GPUBVESolver, and writes u, v, w then the stepped x, y, z, relVort
 !> (float64[n] each) to the output file.  It then calls the PSE and SWE entry
@@ -9,7 +10,12 @@
 !> interpolation of relVort at the first 10 particles, and a solver reloaded
 !> with the same particles and stepped once; it appends gx, gy, gz, u, v, ddot,
 !> lapSurf (n each), the 10 interpolated values and the reloaded solver's
-!> stepped x, y, z, relVort (n each).
+!> stepped x, y, z, relVort (n each).  Last come one FP64 step of the planar
+!> solver on (x + 0.1 z, y + 0.37 z) (x, y, u, v, then KE, enstrophy and
+!> circulation), one FP64 step of a batch holding the sphere problem
+!> (x, y, z, relVort), the learned far-field object's expansion-mode velocity
+!> (near 0.3, band 0.6; u, v, w), and the helpers' PSE eps (0.3^0.75) and
+!> 3-rank MPISetup blocks (start, end).
 program lpm_fortran_smoke
   use iso_c_binding
   use lpm_direct_f
@@ -23,6 +29,13 @@ program lpm_fortran_smoke
   real(c_double), allocatable :: xs(:), ys(:), zs(:), zetas(:), us(:), vs(:), ws(:)
   real(c_double), allocatable :: gx(:), gy(:), gz(:), su(:), sv(:), sdd(:), slap(:), dv(:), hh(:), bot(:)
   real(c_double) :: pint(10)
+  real(c_double), allocatable :: px(:), py(:), pu(:), pv(:)
+  integer(c_int32_t), allocatable :: cell(:)
+  integer(c_int32_t) :: ncell
+  integer(c_int64_t) :: offs(2)
+  integer(c_int) :: istart(3), iend(3), mlen(3)
+  real(c_double) :: diag(3), eps
+  type(c_ptr) :: psolver, batch, ff
   type(GPUBVESolver) :: solver
   integer :: half, unit_in, unit_out
   integer(c_int) :: rc
@@ -34,6 +47,8 @@ program lpm_fortran_smoke
   allocate(x(n), y(n), z(n), zeta(n), area(n), act(n), mask(n), u(n), v(n), w(n))
   allocate(xs(n), ys(n), zs(n), zetas(n), us(n), vs(n), ws(n))
   read(unit_in) x, y, z, zeta, area, act
+  allocate(cell(n))
+  read(unit_in) ncell, cell
   close(unit_in)
   mask = act /= 0
   u = 0.0_c_double
@@ -81,6 +96,46 @@ program lpm_fortran_smoke
   call GetState(solver, xs, ys, zs, zetas, us, vs, ws)
   call Delete(solver)
   write(unit_out) xs, ys, zs, zetas
+
+  ! planar solver (PlaneSolver New / Timestep / Delete) on a projection of the particles
+  allocate(px(n), py(n), pu(n), pv(n))
+  px = x + 0.1_c_double * z
+  py = y + 0.37_c_double * z
+  rc = lpm_plane_solver_new(lpm_context, psolver, n, px, py, zeta, area, act, c_null_ptr, c_null_ptr, &
+                            LPM_PREC_FP64, 0_c_int)
+  if (rc /= LPM_OK) print *, "lpm_plane_solver_new ERROR ", rc
+  rc = lpm_plane_solver_timestep(psolver, 0.01_c_double)
+  if (rc /= LPM_OK) print *, "lpm_plane_solver_timestep ERROR ", rc
+  rc = lpm_plane_solver_get_state(psolver, px, py, pu, pv)
+  rc = lpm_plane_solver_diagnostics(psolver, diag(1), diag(2), diag(3))
+  if (rc /= LPM_OK) print *, "lpm_plane_solver_diagnostics ERROR ", rc
+  rc = lpm_plane_solver_delete(psolver)
+  write(unit_out) px, py, pu, pv, diag
+
+  ! the same sphere problem as a batch of one (offsets 0, n), one FP64 step
+  offs = [0_c_int64_t, int(n, c_int64_t)]
+  rc = lpm_bve_batch_new(lpm_context, batch, 1_c_int, offs, x, y, z, zeta, area, act, [1.0_c_double], &
+                         [2.0_c_double * acos(-1.0_c_double)], LPM_PREC_FP64, 0_c_int)
+  if (rc /= LPM_OK) print *, "lpm_bve_batch_new ERROR ", rc
+  rc = lpm_bve_batch_steps(batch, 0.01_c_double, 1_c_int)
+  if (rc /= LPM_OK) print *, "lpm_bve_batch_steps ERROR ", rc
+  rc = lpm_bve_batch_get_state(batch, xs, ys, zs, zetas, us, vs, ws)
+  rc = lpm_bve_batch_delete(batch)
+  write(unit_out) xs, ys, zs, zetas
+
+  ! learned far-field object, expansion-mode velocity (mode 1 needs no training)
+  rc = lpm_bve_ff_new(lpm_context, ff, n, x, y, z, zeta, area, act, cell, ncell, 1.0_c_double, &
+                      2.0_c_double * acos(-1.0_c_double), 0.3_c_double, 0.6_c_double, 4_c_int)
+  if (rc /= LPM_OK) print *, "lpm_bve_ff_new ERROR ", rc
+  rc = lpm_bve_ff_velocity(ff, 1_c_int, us, vs, ws)
+  if (rc /= LPM_OK) print *, "lpm_bve_ff_velocity ERROR ", rc
+  rc = lpm_bve_ff_delete(ff)
+  write(unit_out) us, vs, ws
+
+  ! helpers: PSE eps and the MPISetup blocks of 3 ranks
+  rc = lpm_pse_eps(0.3_c_double, 0.75_c_double, eps)
+  rc = lpm_load_balance(n, 3_c_int, istart, iend, mlen)
+  write(unit_out) eps, real(istart, c_double), real(iend, c_double)
   close(unit_out)
   print *, "lpm_fortran_smoke: n =", n, " done"
 end program
