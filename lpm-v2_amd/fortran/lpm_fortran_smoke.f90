@@ -14,8 +14,11 @@
 !> solver on (x + 0.1 z, y + 0.37 z) (x, y, u, v, then KE, enstrophy and
 !> circulation), one FP64 step of a batch holding the sphere problem
 !> (x, y, z, relVort), the learned far-field object's expansion-mode velocity
-!> (near 0.3, band 0.6; u, v, w), and the helpers' PSE eps (0.3^0.75) and
-!> 3-rank MPISetup blocks (start, end).
+!> (near 0.3, band 1.5 so that the 20 coarse cells of a level-2 mesh have band
+!> cells to sample; u, v, w), then its sample count, training loss, mode,
+!> last judged error and judge count after 20,000 samples, 2 epochs and two
+!> learned-mode steps judged at every step (tol 1), and the helpers' PSE eps
+!> (0.3^0.75) and 3-rank MPISetup blocks (start, end).
 program lpm_fortran_smoke
   use iso_c_binding
   use lpm_direct_f
@@ -34,7 +37,9 @@ program lpm_fortran_smoke
   integer(c_int32_t) :: ncell
   integer(c_int64_t) :: offs(2)
   integer(c_int) :: istart(3), iend(3), mlen(3)
-  real(c_double) :: diag(3), eps
+  real(c_double) :: diag(3), eps, loss, last_error
+  integer(c_int64_t) :: total
+  integer(c_int) :: mode_out, judged
   type(c_ptr) :: psolver, batch, ff
   type(GPUBVESolver) :: solver
   integer :: half, unit_in, unit_out
@@ -125,12 +130,20 @@ program lpm_fortran_smoke
 
   ! learned far-field object, expansion-mode velocity (mode 1 needs no training)
   rc = lpm_bve_ff_new(lpm_context, ff, n, x, y, z, zeta, area, act, cell, ncell, 1.0_c_double, &
-                      2.0_c_double * acos(-1.0_c_double), 0.3_c_double, 0.6_c_double, 4_c_int)
+                      2.0_c_double * acos(-1.0_c_double), 0.3_c_double, 1.5_c_double, 4_c_int)
   if (rc /= LPM_OK) print *, "lpm_bve_ff_new ERROR ", rc
   rc = lpm_bve_ff_velocity(ff, 1_c_int, us, vs, ws)
   if (rc /= LPM_OK) print *, "lpm_bve_ff_velocity ERROR ", rc
-  rc = lpm_bve_ff_delete(ff)
   write(unit_out) us, vs, ws
+  ! a short training and two judged learned-mode steps (judge at every step)
+  rc = lpm_bve_ff_collect(ff, 20000_c_int, 1_c_int64_t, total)
+  if (rc /= LPM_OK) print *, "lpm_bve_ff_collect ERROR ", rc
+  rc = lpm_bve_ff_train(ff, 2_c_int, 1024_c_int, 1.0d-3, 2_c_int64_t, loss)
+  if (rc /= LPM_OK) print *, "lpm_bve_ff_train ERROR ", rc
+  rc = lpm_bve_ff_steps(ff, 0.01_c_double, 2_c_int, 2_c_int, 1_c_int, 1.0_c_double, mode_out, last_error, judged)
+  if (rc /= LPM_OK) print *, "lpm_bve_ff_steps ERROR ", rc
+  rc = lpm_bve_ff_delete(ff)
+  write(unit_out) real(total, c_double), loss, real(mode_out, c_double), last_error, real(judged, c_double)
 
   ! helpers: PSE eps and the MPISetup blocks of 3 ranks
   rc = lpm_pse_eps(0.3_c_double, 0.75_c_double, eps)

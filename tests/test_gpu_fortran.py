@@ -12,7 +12,7 @@ import pytest
 
 from helpers import icos, rel_err_max
 from lpmgpu import ic
-from lpmgpu.farfield import EXPANSION, FarFieldBVE
+from lpmgpu.farfield import EXPANSION, LEARNED, FarFieldBVE
 
 pytestmark = pytest.mark.gpu
 
@@ -41,7 +41,7 @@ def test_fortran_drop_in(ctx, oracle, tmp_path):
     print(r.stdout, r.stderr)
     assert r.returncode == 0 and "ERROR" not in r.stdout
     raw = np.fromfile(out, np.float64)
-    assert raw.size == 29 * n + 20
+    assert raw.size == 29 * n + 25
     res = raw[:7 * n].reshape(7, n)
     ref = oracle.bve_sphere_velocity(m.x, m.y, m.z, zeta, m.area, m.active)
     assert rel_err_max(tuple(res[:3]), ref) < 1e-11
@@ -85,14 +85,20 @@ def test_fortran_drop_in(ctx, oracle, tmp_path):
     # the learned far-field object in expansion mode, as from Python
     k += 4 * n
     ffv = raw[k: k + 3 * n].reshape(3, n)
-    f = FarFieldBVE(ctx, m, zeta, rotation_rate=2 * np.pi, near_radius=0.3, band_radius=0.6, order=4)
+    f = FarFieldBVE(ctx, m, zeta, rotation_rate=2 * np.pi, near_radius=0.3, band_radius=1.5, order=4)
     want = f.velocity(EXPANSION)
     f.delete()
     assert rel_err_max(tuple(ffv), want) < 1e-12
     assert rel_err_max(tuple(ffv), ref) < 1e-2  # the expansion approximates the direct sum
 
-    # helpers
+    # its training and judged learned-mode steps, driven from Fortran
     k += 3 * n
+    total, loss, mode, last_error, judged = raw[k: k + 5]
+    assert total == 20000 and np.isfinite(loss) and loss > 0
+    assert mode == LEARNED and judged == 2 and 0 <= last_error < 1.0
+
+    # helpers
+    k += 5
     assert raw[k] == oracle.pse_eps(0.3, 0.75)
     s, e, _ = oracle.load_balance(n, 3)
     assert np.array_equal(raw[k + 1: k + 4], s) and np.array_equal(raw[k + 4: k + 7], e)
