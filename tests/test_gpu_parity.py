@@ -361,6 +361,38 @@ def test_invalid_arguments_report_errors(ctx):
     assert np.isfinite(np.array(u)).all()
 
 
+def test_empty_blocks_and_no_particles(ctx):
+    """An empty target block [i0, i0) leaves every output untouched (zeros from
+    the wrapper) and n = 0 returns empty arrays, for each drop-in sum, with the
+    context usable afterwards."""
+    m, zeta = cfg0_inputs(1)
+    absv = zeta + 2.0 * 2 * np.pi * m.z
+    e = np.zeros(0)
+    none = np.zeros(0, np.int32)
+    for prec in ("fp64", "fp32"):
+        calls = (
+            lambda **b: ctx.bve_sphere_velocity(m.x, m.y, m.z, zeta, m.area, m.active, precision=prec, **b),
+            lambda **b: ctx.plane_velocity(m.x, m.y, zeta, m.area, m.active, precision=prec, **b),
+            lambda **b: ctx.pse_plane_laplacian(m.x, m.y, zeta, m.area, m.active, 0.1, precision=prec, **b),
+            lambda **b: ctx.bve_sphere_stream(m.x, m.y, m.z, zeta, absv, m.area, m.active, precision=prec, **b),
+            lambda **b: ctx.plane_stream(m.x, m.y, zeta, m.area, m.active, precision=prec, **b),
+        )
+        for k, call in enumerate(calls):
+            out = np.array(call(i0=7, i1=7))
+            assert out.shape[-1] == m.n and not out.any(), (prec, k)
+        empty = (
+            ctx.bve_sphere_velocity(e, e, e, e, e, none, precision=prec),
+            ctx.plane_velocity(e, e, e, e, none, precision=prec),
+            ctx.pse_plane_laplacian(e, e, e, e, none, 0.1, precision=prec),
+            ctx.bve_sphere_stream(e, e, e, e, e, e, none, precision=prec),
+            ctx.plane_stream(e, e, e, e, none, precision=prec),
+        )
+        for k, out in enumerate(empty):
+            assert np.array(out).size == 0, (prec, k)
+    u = ctx.bve_sphere_velocity(m.x, m.y, m.z, zeta, m.area, m.active, precision="fp64")
+    assert np.isfinite(np.array(u)).all() and np.array(u).any()
+
+
 def test_no_active_sources_gives_zero(ctx):
     m, zeta = cfg0_inputs(1)
     u = ctx.bve_sphere_velocity(m.x, m.y, m.z, zeta, m.area, np.zeros(m.n, np.int32), precision="fp32")
