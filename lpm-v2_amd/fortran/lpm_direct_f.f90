@@ -518,6 +518,130 @@ module lpm_direct_f
       type(c_ptr), value :: ctx
       integer(c_int), value :: kind, precision, targets_per_lane, tile_sources, splits
     end function
+
+    integer(c_int) function lpm_get_tuning(ctx, kind, precision, targets_per_lane, tile_sources, splits) &
+        bind(C, name="lpm_get_tuning")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: kind, precision
+      integer(c_int), intent(out) :: targets_per_lane, tile_sources, splits
+    end function
+
+    integer(c_int) function lpm_reset_kernel_stats(ctx) bind(C, name="lpm_reset_kernel_stats")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+    end function
+
+    integer(c_int) function lpm_synchronize(ctx) bind(C, name="lpm_synchronize")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+    end function
+
+    ! batched solver: launch shape, work items and pairs per evaluation
+    integer(c_int) function lpm_bve_batch_set_tuning(batch, targets_per_lane, tile_sources, chunk_sources) &
+        bind(C, name="lpm_bve_batch_set_tuning")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: batch
+      integer(c_int), value :: targets_per_lane, tile_sources, chunk_sources
+    end function
+
+    integer(c_int) function lpm_bve_batch_get_tuning(batch, targets_per_lane, tile_sources, chunk_sources, &
+        n_items) bind(C, name="lpm_bve_batch_get_tuning")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: batch
+      integer(c_int), intent(out) :: targets_per_lane, tile_sources, chunk_sources, n_items
+    end function
+
+    integer(c_int) function lpm_bve_batch_pairs(batch, pairs_per_eval) bind(C, name="lpm_bve_batch_pairs")
+      import :: c_int, c_ptr, c_int64_t
+      type(c_ptr), value :: batch
+      integer(c_int64_t), intent(out) :: pairs_per_eval
+    end function
+
+    ! learned far field: particle cells (length n), near/band cell counts
+    ! (length ncell each) and stored training samples
+    integer(c_int) function lpm_bve_ff_particle_cells(ff, cell) bind(C, name="lpm_bve_ff_particle_cells")
+      import :: c_int, c_ptr, c_int32_t
+      type(c_ptr), value :: ff
+      integer(c_int32_t) :: cell(*)
+    end function
+
+    integer(c_int) function lpm_bve_ff_cells(ff, near_cells, band_cells) bind(C, name="lpm_bve_ff_cells")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ff
+      integer(c_int) :: near_cells(*), band_cells(*)
+    end function
+
+    integer(c_int) function lpm_bve_ff_samples(ff, offset, count, inputs, targets, ids) &
+        bind(C, name="lpm_bve_ff_samples")
+      import :: c_int, c_ptr, c_int64_t, c_float, c_int32_t
+      type(c_ptr), value :: ff
+      integer(c_int64_t), value :: offset, count
+      real(c_float) :: inputs(*), targets(*)
+      integer(c_int32_t) :: ids(*)
+    end function
+
+    ! stand-alone MLP (FP32 parameters; BF16 MFMA or FP32 forward)
+    integer(c_int) function lpm_mlp_new(ctx, mlp, nin, width, nhidden, nout, activation, seed) &
+        bind(C, name="lpm_mlp_new")
+      import :: c_int, c_ptr, c_int64_t
+      type(c_ptr), value :: ctx
+      type(c_ptr), intent(out) :: mlp
+      integer(c_int), value :: nin, width, nhidden, nout, activation
+      integer(c_int64_t), value :: seed
+    end function
+
+    integer(c_int) function lpm_mlp_delete(mlp) bind(C, name="lpm_mlp_delete")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: mlp
+    end function
+
+    integer(c_int) function lpm_mlp_param_count(mlp, count) bind(C, name="lpm_mlp_param_count")
+      import :: c_int, c_ptr, c_int64_t
+      type(c_ptr), value :: mlp
+      integer(c_int64_t), intent(out) :: count
+    end function
+
+    integer(c_int) function lpm_mlp_set_params(mlp, params, count) bind(C, name="lpm_mlp_set_params")
+      import :: c_int, c_ptr, c_float, c_int64_t
+      type(c_ptr), value :: mlp
+      real(c_float) :: params(*)
+      integer(c_int64_t), value :: count
+    end function
+
+    integer(c_int) function lpm_mlp_get_params(mlp, params, count) bind(C, name="lpm_mlp_get_params")
+      import :: c_int, c_ptr, c_float, c_int64_t
+      type(c_ptr), value :: mlp
+      real(c_float) :: params(*)
+      integer(c_int64_t), value :: count
+    end function
+
+    integer(c_int) function lpm_mlp_forward(mlp, n, inputs, outputs, use_mfma) bind(C, name="lpm_mlp_forward")
+      import :: c_int, c_ptr, c_float
+      type(c_ptr), value :: mlp
+      integer(c_int), value :: n, use_mfma
+      real(c_float) :: inputs(*), outputs(*)
+    end function
+
+    integer(c_int) function lpm_mlp_loss_grad(mlp, n, inputs, targets, grad, loss) &
+        bind(C, name="lpm_mlp_loss_grad")
+      import :: c_int, c_ptr, c_float, c_double
+      type(c_ptr), value :: mlp
+      integer(c_int), value :: n
+      real(c_float) :: inputs(*), targets(*), grad(*)
+      real(c_double), intent(out) :: loss
+    end function
+
+    integer(c_int) function lpm_mlp_train(mlp, n, inputs, targets, epochs, batch, lr, seed, final_loss) &
+        bind(C, name="lpm_mlp_train")
+      import :: c_int, c_ptr, c_float, c_double, c_int64_t
+      type(c_ptr), value :: mlp
+      integer(c_int), value :: n, epochs, batch
+      real(c_float) :: inputs(*), targets(*)
+      real(c_double), value :: lr
+      integer(c_int64_t), value :: seed
+      real(c_double), intent(out) :: final_loss
+    end function
   end interface
 
 contains

@@ -18,7 +18,10 @@
 !> cells to sample; u, v, w), then its sample count, training loss, mode,
 !> last judged error and judge count after 20,000 samples, 2 epochs and two
 !> learned-mode steps judged at every step (tol 1), and the helpers' PSE eps
-!> (0.3^0.75) and 3-rank MPISetup blocks (start, end).
+!> (0.3^0.75) and 3-rank MPISetup blocks (start, end).  Last, the batch's pair
+!> count per evaluation and the number of particles whose cell, as the
+!> far-field object binned them at its last evaluation, differs from the input
+!> cell.
 program lpm_fortran_smoke
   use iso_c_binding
   use lpm_direct_f
@@ -33,7 +36,8 @@ program lpm_fortran_smoke
   real(c_double), allocatable :: gx(:), gy(:), gz(:), su(:), sv(:), sdd(:), slap(:), dv(:), hh(:), bot(:)
   real(c_double) :: pint(10)
   real(c_double), allocatable :: px(:), py(:), pu(:), pv(:)
-  integer(c_int32_t), allocatable :: cell(:)
+  integer(c_int32_t), allocatable :: cell(:), cell2(:)
+  integer(c_int64_t) :: npairs
   integer(c_int32_t) :: ncell
   integer(c_int64_t) :: offs(2)
   integer(c_int) :: istart(3), iend(3), mlen(3)
@@ -125,6 +129,8 @@ program lpm_fortran_smoke
   rc = lpm_bve_batch_steps(batch, 0.01_c_double, 1_c_int)
   if (rc /= LPM_OK) print *, "lpm_bve_batch_steps ERROR ", rc
   rc = lpm_bve_batch_get_state(batch, xs, ys, zs, zetas, us, vs, ws)
+  rc = lpm_bve_batch_pairs(batch, npairs)
+  if (rc /= LPM_OK) print *, "lpm_bve_batch_pairs ERROR ", rc
   rc = lpm_bve_batch_delete(batch)
   write(unit_out) xs, ys, zs, zetas
 
@@ -142,6 +148,9 @@ program lpm_fortran_smoke
   if (rc /= LPM_OK) print *, "lpm_bve_ff_train ERROR ", rc
   rc = lpm_bve_ff_steps(ff, 0.01_c_double, 2_c_int, 2_c_int, 1_c_int, 1.0_c_double, mode_out, last_error, judged)
   if (rc /= LPM_OK) print *, "lpm_bve_ff_steps ERROR ", rc
+  allocate(cell2(n))
+  rc = lpm_bve_ff_particle_cells(ff, cell2)
+  if (rc /= LPM_OK) print *, "lpm_bve_ff_particle_cells ERROR ", rc
   rc = lpm_bve_ff_delete(ff)
   write(unit_out) real(total, c_double), loss, real(mode_out, c_double), last_error, real(judged, c_double)
 
@@ -149,6 +158,9 @@ program lpm_fortran_smoke
   rc = lpm_pse_eps(0.3_c_double, 0.75_c_double, eps)
   rc = lpm_load_balance(n, 3_c_int, istart, iend, mlen)
   write(unit_out) eps, real(istart, c_double), real(iend, c_double)
+  rc = lpm_synchronize(lpm_context)
+  if (rc /= LPM_OK) print *, "lpm_synchronize ERROR ", rc
+  write(unit_out) real(npairs, c_double), real(count(cell2 /= cell), c_double)
   close(unit_out)
   print *, "lpm_fortran_smoke: n =", n, " done"
 end program
