@@ -1643,6 +1643,9 @@ int lpm_bve_ff_set_state(lpm_bve_ff* f, const double* x, const double* y, const 
 int lpm_bve_ff_particle_cells(lpm_bve_ff* f, int32_t* cell) {
   if (!f || !cell) return LPM_ERR_ARG;
   lpm_ctx* ctx = f->ctx;
+  ctx->err.clear();
+  if (int rc = ff_usable(f)) return rc;
+  LPM_HIP(ctx, hipSetDevice(ctx->device));
   LPM_HIP(ctx, hipMemcpyAsync(cell, f->cellof.p, (size_t)f->n * 4, hipMemcpyDeviceToHost, ctx->stream));
   LPM_HIP(ctx, hipStreamSynchronize(ctx->stream));
   return LPM_OK;
@@ -1651,6 +1654,9 @@ int lpm_bve_ff_particle_cells(lpm_bve_ff* f, int32_t* cell) {
 int lpm_bve_ff_cells(lpm_bve_ff* f, int* near_cells, int* band_cells) {
   if (!f || !near_cells || !band_cells) return LPM_ERR_ARG;
   lpm_ctx* ctx = f->ctx;
+  ctx->err.clear();
+  if (int rc = ff_usable(f)) return rc;
+  LPM_HIP(ctx, hipSetDevice(ctx->device));
   LPM_HIP(ctx, hipMemcpyAsync(near_cells, f->near_count.p, (size_t)f->ncell * 4, hipMemcpyDeviceToHost, ctx->stream));
   LPM_HIP(ctx, hipMemcpyAsync(band_cells, f->band_count.p, (size_t)f->ncell * 4, hipMemcpyDeviceToHost, ctx->stream));
   LPM_HIP(ctx, hipStreamSynchronize(ctx->stream));

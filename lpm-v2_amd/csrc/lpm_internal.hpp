@@ -127,6 +127,7 @@ struct PairProblem {
   DevBuf* tgt_buf = nullptr;
   DevBuf* src_buf = nullptr;
   DevBuf* partial_buf = nullptr;
+  DevBuf* red_buf = nullptr;  // FP32 sphere Green: the two FP64 source totals
 };
 
 // Epilogues of the PSE derivative kinds.

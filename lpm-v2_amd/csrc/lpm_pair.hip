@@ -577,10 +577,11 @@ int run_pair_sum(lpm_ctx* ctx, const PairProblem& pp) {
   if (kind == LPM_K_PSE_INTERP_PLANE || kind == LPM_K_PSE_INTERP_SPHERE) f.scale = 1.0 / (kPi * pp.eps * pp.eps);
   if (kind == LPM_K_SWE_PLANE) f.scale = 1.0 / (kPi * e4);
   if (kind == LPM_K_SPHERE_GREEN && f.fp32) {
-    LPM_HIP(ctx, ctx->red.ensure(2 * sizeof(double)));
+    DevBuf& rb = pp.red_buf ? *pp.red_buf : ctx->red;
+    LPM_HIP(ctx, rb.ensure(2 * sizeof(double)));
     hipLaunchKernelGGL(source_totals, dim3(1), dim3(256), 0, ctx->stream, ns, pp.src_list, pp.q0,
-                       pp.q1, pp.area, ctx->red.as<double>());
-    f.G = ctx->red.as<double>();
+                       pp.q1, pp.area, rb.as<double>());
+    f.G = rb.as<double>();
   }
   if (ops) {
     if (ops_launch_finalize(kind, pp.m, f, ctx->stream) != LPM_OK)

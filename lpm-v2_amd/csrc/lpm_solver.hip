@@ -166,10 +166,14 @@ struct lpm_bve_solver {
   DevBuf xi, yi, zi, qi;
   DevBuf psi_rel, psi_abs;
   DevBuf tgt, src, partial;
+  DevBuf tot;   // FP32 stream functions: FP64 source totals
   DevBuf ptrs;  // device pointer tables for rk4_combine
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   double graph_dt = 0.0;
+  // addresses of the workspaces a captured step uses, as captured; an on-demand
+  // stream-function call may regrow them, and then the step is captured again
+  void* graph_bufs[6] = {};
   bool has_abs = false;
   bool loaded = false;  // false after a failed particle reload
 };
@@ -229,6 +233,7 @@ int bve_stream(lpm_bve_solver* s) {
   pp.tgt_buf = &s->tgt;
   pp.src_buf = &s->src;
   pp.partial_buf = &s->partial;
+  pp.red_buf = &s->tot;
   return lpm::run_pair_sum(ctx, pp);
 }
 
@@ -266,11 +271,27 @@ int bve_enqueue_step(lpm_bve_solver* s, double dt) {
   return LPM_OK;
 }
 
+// The workspaces a step grows on demand (the state and stage arrays change
+// only in bve_load, which drops the graph).
+void bve_work_bufs(const lpm_bve_solver* s, void* out[6]) {
+  const DevBuf* b[6] = {&s->tgt, &s->src, &s->partial, &s->psi_rel, &s->psi_abs, &s->tot};
+  for (int k = 0; k < 6; ++k) out[k] = b[k]->p;
+}
+
+template <class Solver, class BufsFn>
+bool graph_bufs_unchanged(const Solver* s, BufsFn bufs) {
+  void* now[6];
+  bufs(s, now);
+  for (int k = 0; k < 6; ++k)
+    if (now[k] != s->graph_bufs[k]) return false;
+  return true;
+}
+
 int bve_step(lpm_bve_solver* s, double dt) {
   lpm_ctx* ctx = s->ctx;
   const bool use_graph = (s->flags & LPM_SOLVER_USE_GRAPH) && !ctx->timing;
   if (!use_graph) return bve_enqueue_step(s, dt);
-  if (s->exec && s->graph_dt == dt) {
+  if (s->exec && s->graph_dt == dt && graph_bufs_unchanged(s, bve_work_bufs)) {
     LPM_HIP(ctx, hipGraphLaunch(s->exec, ctx->stream));
     return LPM_OK;
   }
@@ -297,6 +318,7 @@ int bve_step(lpm_bve_solver* s, double dt) {
   s->graph = gr;
   LPM_HIP(ctx, hipGraphInstantiate(&s->exec, s->graph, nullptr, nullptr, 0));
   s->graph_dt = dt;
+  bve_work_bufs(s, s->graph_bufs);
   return LPM_OK;
 }
 
@@ -501,7 +523,7 @@ int lpm_bve_solver_delete(lpm_bve_solver* s) {
   if (s->graph) (void)hipGraphDestroy(s->graph);
   DevBuf* all[] = {&s->x, &s->y, &s->z, &s->zeta, &s->absv, &s->u, &s->v, &s->w, &s->area,
                    &s->active, &s->list, &s->xi, &s->yi, &s->zi, &s->qi, &s->psi_rel, &s->psi_abs,
-                   &s->tgt, &s->src, &s->partial, &s->ptrs};
+                   &s->tgt, &s->src, &s->partial, &s->tot, &s->ptrs};
   for (DevBuf* b : all) b->release();
   for (auto& row : s->k)
     for (auto& b : row) b.release();
@@ -524,6 +546,7 @@ struct lpm_plane_solver {
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   double graph_dt = 0.0;
+  void* graph_bufs[6] = {};  // workspace addresses as captured (see lpm_bve_solver)
   bool loaded = false;  // false after a failed particle reload
 };
 
@@ -597,11 +620,17 @@ int plane_enqueue_step(lpm_plane_solver* s, double dt) {
   return LPM_OK;
 }
 
+void plane_work_bufs(const lpm_plane_solver* s, void* out[6]) {
+  const DevBuf* b[4] = {&s->tgt, &s->src, &s->partial, &s->psi};
+  for (int k = 0; k < 4; ++k) out[k] = b[k]->p;
+  out[4] = out[5] = nullptr;
+}
+
 int plane_step(lpm_plane_solver* s, double dt) {
   lpm_ctx* ctx = s->ctx;
   const bool use_graph = (s->flags & LPM_SOLVER_USE_GRAPH) && !ctx->timing;
   if (!use_graph) return plane_enqueue_step(s, dt);
-  if (s->exec && s->graph_dt == dt) {
+  if (s->exec && s->graph_dt == dt && graph_bufs_unchanged(s, plane_work_bufs)) {
     LPM_HIP(ctx, hipGraphLaunch(s->exec, ctx->stream));
     return LPM_OK;
   }
@@ -627,6 +656,7 @@ int plane_step(lpm_plane_solver* s, double dt) {
   s->graph = gr;
   LPM_HIP(ctx, hipGraphInstantiate(&s->exec, s->graph, nullptr, nullptr, 0));
   s->graph_dt = dt;
+  plane_work_bufs(s, s->graph_bufs);
   return LPM_OK;
 }
 

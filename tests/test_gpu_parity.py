@@ -318,6 +318,57 @@ def test_bve_solver_graph_replay_matches_eager(ctx):
     b.delete()
 
 
+def test_bve_solver_graph_survives_diagnostics_and_stream_calls():
+    """A captured step graph with stream functions in every step, interleaved
+    with diagnostics and on-demand stream-function calls on the same context
+    (which grow the context's reduction buffer and may regrow the solver's
+    workspaces), stays bitwise equal to an eager solver doing the same calls.
+    A fresh context, so its buffers start small."""
+    import lpmgpu
+    from lpmgpu import BVESolver
+
+    m, zeta = cfg0_inputs(4)
+    absv = zeta + 2 * (2 * np.pi) * m.z
+    with lpmgpu.Context(0) as c:
+        kw = dict(absvort=absv, rotation_rate=2 * np.pi, precision="fp32", stream_each_step=True, project=True)
+        a = BVESolver(c, m.x, m.y, m.z, zeta, m.area, m.active, **kw)
+        b = BVESolver(c, m.x, m.y, m.z, zeta, m.area, m.active, use_graph=True, **kw)
+        for k, act in enumerate(["diag", "stream", "diag", "none", "stream"]):
+            a.timestep(0.01)
+            b.timestep(0.01)
+            if act == "diag":
+                assert a.diagnostics() == b.diagnostics()
+            elif act == "stream":
+                pa, pb = a.stream(), b.stream()
+                assert all(np.array_equal(u, v) for u, v in zip(pa, pb))
+            sa, sb = a.state(), b.state()
+            for key in sa:
+                assert np.array_equal(sa[key], sb[key]), (k, key)
+        a.delete()
+        b.delete()
+
+
+def test_plane_solver_graph_survives_stream_calls():
+    import lpmgpu
+    from lpmgpu import PlaneSolver
+
+    m, vort = _plane_case(3)
+    with lpmgpu.Context(0) as c:
+        a = PlaneSolver(c, m.x, m.y, vort, m.area, m.active, precision="fp32", stream_each_step=True)
+        b = PlaneSolver(c, m.x, m.y, vort, m.area, m.active, precision="fp32", stream_each_step=True,
+                        use_graph=True)
+        for k in range(4):
+            a.timestep(0.005)
+            b.timestep(0.005)
+            assert a.diagnostics() == b.diagnostics()
+            assert np.array_equal(a.stream(), b.stream())
+            sa, sb = a.state(), b.state()
+            for key in sa:
+                assert np.array_equal(sa[key], sb[key]), (k, key)
+        a.delete()
+        b.delete()
+
+
 @pytest.mark.parametrize("prec,tol", [("fp64", 1e-11), ("fp32", 1e-6)])
 def test_plane_solver_matches_oracle_steps(ctx, oracle, prec, tol):
     m, vort = _plane_case(3)
