@@ -84,6 +84,28 @@ int lpm_bve_sphere_velocity(lpm_ctx *ctx, double *u, double *v, double *w, const
                             const double *area, double radius, double rotation_rate,
                             const int32_t *active, int n, int i0, int i1, int precision);
 
+/* Allocation-free variant of lpm_bve_sphere_velocity.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code next
+ * to HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * Same arguments and result as lpm_bve_sphere_velocity, except that every
+ * array is DEVICE memory of the context's device (x, y, z, relvort, area,
+ * active of length n; u, v, w of length n, written for targets [i0, i1)).
+ * The active list, the pair records and the split partials live in a static
+ * 64 MiB device array that each GPU holds for the life of the process, so
+ * the call allocates nothing; n is limited to 131,072 particles
+ * (LPM_NOALLOC_MAX_N; larger n fails with LPM_ERR_ARG).  The array is shared
+ * by every call on one GPU (all contexts of that device) and keeps the last
+ * call's contents: calls that overlap on one GPU race.  The call runs on the
+ * context's stream (inputs must be complete when it is made) and returns
+ * when the outputs are written.  Source splits are sized from n rather than
+ * the active count, so FP32 results agree with lpm_bve_sphere_velocity to
+ * rounding, not bitwise. */
+#define LPM_NOALLOC_MAX_N 131072
+int lpm_bve_sphere_velocity_noalloc(lpm_ctx *ctx, double *u, double *v, double *w, const double *x,
+                                    const double *y, const double *z, const double *relvort,
+                                    const double *area, double radius, double rotation_rate,
+                                    const int32_t *active, int n, int i0, int i1, int precision);
+
 /* Planar Biot-Savart velocity (singular point vortices).  Replaces
  * planarIncompressibleVelocity (PlaneIncompressibleSolver.f90:278-316) and
  * PlanarIncompressible.f90:426-466.

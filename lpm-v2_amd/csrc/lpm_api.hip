@@ -5,14 +5,94 @@
 #include <string>
 
 #include "lpm_internal.hpp"
+#include "pair_kernels.hpp"
 
 using lpm::DevBuf;
 using lpm::fail;
 using lpm::hip_fail;
 
+// Workspace of the allocation-free sphere Biot-Savart entry point
+// (lpm_bve_sphere_velocity_noalloc): the active-source list and its counts,
+// the target and source records and the FP64 split partials of one call.
+// Each GPU the library runs on holds its own copy for the life of the
+// process; host code reaches it only through the address that lpm_create
+// resolves per device (ctx->scratch), never through this name.
+constexpr size_t kScratchBytes = size_t(64) << 20;
+constexpr int kNoAllocMaxN = 131072;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
 namespace {
 
 thread_local std::string g_create_error;
+
+// Active-particle list from device flags, in two passes over blocks of 1024
+// particles: per-block counts, then each block's ascending indices at its
+// prefix offset (deterministic; the same list the host-side compaction builds).
+constexpr int kCompactBlock = 1024;
+
+__global__ __launch_bounds__(kCompactBlock) void count_active_blocks(int n, const int32_t* __restrict__ active,
+                                                                     int* __restrict__ block_count) {
+  __shared__ int wsum[kCompactBlock / 64];
+  const int i = blockIdx.x * kCompactBlock + threadIdx.x;
+  const bool a = i < n && active[i] != 0;
+  const unsigned long long m = __ballot(a);
+  if ((threadIdx.x & 63) == 0) wsum[threadIdx.x >> 6] = __popcll(m);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int t = 0;
+    for (int k = 0; k < kCompactBlock / 64; ++k) t += wsum[k];
+    block_count[blockIdx.x] = t;
+  }
+}
+
+__global__ __launch_bounds__(kCompactBlock) void write_active_list(int n, const int32_t* __restrict__ active,
+                                                                   const int* __restrict__ block_count,
+                                                                   int* __restrict__ list, int* __restrict__ ns) {
+  __shared__ int wsum[kCompactBlock / 64];
+  __shared__ int base;
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  if (threadIdx.x == 0) {
+    int b = 0;
+    for (int k = 0; k < (int)blockIdx.x; ++k) b += block_count[k];
+    base = b;
+    if (blockIdx.x == gridDim.x - 1) *ns = b + block_count[blockIdx.x];
+  }
+  const int i = blockIdx.x * kCompactBlock + threadIdx.x;
+  const bool a = i < n && active[i] != 0;
+  const unsigned long long m = __ballot(a);
+  if (lane == 0) wsum[wid] = __popcll(m);
+  __syncthreads();
+  if (a) {
+    int off = base;
+    for (int k = 0; k < wid; ++k) off += wsum[k];
+    off += __popcll(m & ((1ull << lane) - 1ull));
+    list[off] = i;
+  }
+}
+
+// Device-memory check for the allocation-free entry point: the pointer must
+// be device (or managed) memory of this context's device, with `bytes`
+// bytes inside its allocation.
+int check_device_array(lpm_ctx* ctx, const void* p, size_t bytes, const char* name) {
+  const std::string nm = std::string("lpm_bve_sphere_velocity_noalloc: ") + name;
+  hipPointerAttribute_t at;
+  if (hipPointerGetAttributes(&at, p) != hipSuccess) {
+    (void)hipGetLastError();
+    return fail(ctx, LPM_ERR_ARG, nm + " is not device memory");
+  }
+  if (at.type == hipMemoryTypeManaged) return LPM_OK;
+  if (at.type != hipMemoryTypeDevice) return fail(ctx, LPM_ERR_ARG, nm + " is not device memory");
+  if (at.device != ctx->device) return fail(ctx, LPM_ERR_ARG, nm + " is memory of another device");
+  hipDeviceptr_t base = nullptr;
+  size_t size = 0;
+  if (hipMemGetAddressRange(&base, &size, const_cast<void*>(p)) != hipSuccess) {
+    (void)hipGetLastError();
+    return fail(ctx, LPM_ERR_ARG, nm + ": allocation range unknown");
+  }
+  if ((const char*)p + bytes > (const char*)base + size)
+    return fail(ctx, LPM_ERR_ARG, nm + " is shorter than n elements");
+  return LPM_OK;
+}
 
 int check_block(lpm_ctx* ctx, int n, int i0, int i1) {
   if (n < 0) return fail(ctx, LPM_ERR_ARG, "n must be >= 0");
@@ -184,6 +264,14 @@ int lpm_create(lpm_ctx** out, int device) {
     delete ctx;
     return LPM_ERR_HIP;
   }
+  // this device's copy of the static scratch array (device current above)
+  e = hipGetSymbolAddress(&ctx->scratch, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || !ctx->scratch) {
+    g_create_error = std::string("lpm_create: hipGetSymbolAddress(scratch) failed: ") + hipGetErrorString(e);
+    (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return LPM_ERR_HIP;
+  }
   g_create_error.clear();
   *out = ctx;
   return LPM_OK;
@@ -258,6 +346,84 @@ int lpm_bve_sphere_velocity(lpm_ctx* ctx, double* u, double* v, double* w, const
   pp.out2 = ctx->out[2].as<double>();
   if (int rc = lpm::run_pair_sum(ctx, pp)) return rc;
   return download(ctx, i0, i1, {u, v, w});
+}
+
+int lpm_bve_sphere_velocity_noalloc(lpm_ctx* ctx, double* u, double* v, double* w, const double* x,
+                                    const double* y, const double* z, const double* relvort,
+                                    const double* area, double radius, double rotation_rate,
+                                    const int32_t* active, int n, int i0, int i1, int precision) {
+  (void)rotation_rate;
+  if (!ctx) return LPM_ERR_ARG;
+  ctx->err.clear();
+  if (int rc = check_block(ctx, n, i0, i1)) return rc;
+  if (int rc = check_prec(ctx, precision)) return rc;
+  if (!u || !v || !w || !x || !y || !z || !relvort || !area || !active)
+    return fail(ctx, LPM_ERR_ARG, "lpm_bve_sphere_velocity_noalloc: null array argument");
+  if (!(radius > 0.0)) return fail(ctx, LPM_ERR_ARG, "lpm_bve_sphere_velocity_noalloc: radius must be > 0");
+  if (n > kNoAllocMaxN)
+    return fail(ctx, LPM_ERR_ARG,
+                "lpm_bve_sphere_velocity_noalloc: n = " + std::to_string(n) + " exceeds the scratch limit of " +
+                    std::to_string(kNoAllocMaxN) + " particles");
+  LPM_HIP(ctx, hipSetDevice(ctx->device));
+  if (i1 == i0) return LPM_OK;
+  const size_t nb = (size_t)n * sizeof(double);
+  const void* ins[6] = {x, y, z, relvort, area, active};
+  const size_t in_bytes[6] = {nb, nb, nb, nb, nb, (size_t)n * sizeof(int32_t)};
+  const char* in_names[6] = {"x", "y", "z", "relvort", "area", "active"};
+  for (int k = 0; k < 6; ++k)
+    if (int rc = check_device_array(ctx, ins[k], in_bytes[k], in_names[k])) return rc;
+  double* outs[3] = {u, v, w};
+  const char* out_names[3] = {"u", "v", "w"};
+  for (int k = 0; k < 3; ++k)
+    if (int rc = check_device_array(ctx, outs[k], nb, out_names[k])) return rc;
+
+  // scratch layout: [block counts | source count] [list] [target records]
+  // [source records] [split partials: the rest]
+  auto up = [](size_t b) { return (b + 255) / 256 * 256; };
+  const int nblk = (n + kCompactBlock - 1) / kCompactBlock;
+  const size_t wsz = precision == LPM_PREC_FP32_KAHAN ? 4 : 8;
+  const size_t off_list = up((size_t)(nblk + 1) * sizeof(int));
+  const size_t off_tgt = off_list + up((size_t)n * sizeof(int));
+  const size_t off_src = off_tgt + up((size_t)(i1 - i0) * lpm::TREC * wsz);
+  const size_t off_part = off_src + up((size_t)n * lpm::SREC * wsz);
+  if (off_part >= kScratchBytes)
+    return fail(ctx, LPM_ERR_ARG, "lpm_bve_sphere_velocity_noalloc: n too large for the scratch array");
+  unsigned char* base = static_cast<unsigned char*>(ctx->scratch);
+  int* block_count = reinterpret_cast<int*>(base);
+  int* ns_dev = block_count + nblk;
+  int* list = reinterpret_cast<int*>(base + off_list);
+  DevBuf tb = DevBuf::view(base + off_tgt, off_src - off_tgt);
+  DevBuf sb = DevBuf::view(base + off_src, off_part - off_src);
+  DevBuf pb = DevBuf::view(base + off_part, kScratchBytes - off_part);
+
+  hipLaunchKernelGGL(count_active_blocks, dim3(nblk), dim3(kCompactBlock), 0, ctx->stream, n, active, block_count);
+  hipLaunchKernelGGL(write_active_list, dim3(nblk), dim3(kCompactBlock), 0, ctx->stream, n, active, block_count,
+                     list, ns_dev);
+  LPM_HIP(ctx, hipGetLastError());
+  lpm::PairProblem pp;
+  pp.kind = LPM_K_SPHERE_BS;
+  pp.precision = precision;
+  pp.n = n;
+  pp.i0 = i0;
+  pp.i1 = i1;
+  pp.x = x;
+  pp.y = y;
+  pp.z = z;
+  pp.q0 = relvort;
+  pp.area = area;
+  pp.src_list = list;
+  pp.ns = n;  // bound; the kernels read the count from ns_dev
+  pp.ns_dev = ns_dev;
+  pp.radius = radius;
+  pp.out0 = u;
+  pp.out1 = v;
+  pp.out2 = w;
+  pp.tgt_buf = &tb;
+  pp.src_buf = &sb;
+  pp.partial_buf = &pb;
+  if (int rc = lpm::run_pair_sum(ctx, pp)) return rc;
+  LPM_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return LPM_OK;
 }
 
 int lpm_plane_velocity(lpm_ctx* ctx, double* u, double* v, const double* x, const double* y,

@@ -13,12 +13,23 @@
 
 namespace lpm {
 
-// Grow-only device buffer owned by a context or solver.
+// Grow-only device buffer owned by a context or solver.  A `fixed` buffer is
+// a view of memory owned elsewhere (a segment of the static scratch array):
+// it never allocates or frees, and a request beyond its size fails.
 struct DevBuf {
   void* p = nullptr;
   size_t bytes = 0;
+  bool fixed = false;
+  static DevBuf view(void* ptr, size_t n) {
+    DevBuf b;
+    b.p = ptr;
+    b.bytes = n;
+    b.fixed = true;
+    return b;
+  }
   hipError_t ensure(size_t n) {
     if (n <= bytes) return hipSuccess;
+    if (fixed) return hipErrorOutOfMemory;
     if (p) {
       hipError_t e = hipFree(p);
       p = nullptr;
@@ -30,7 +41,7 @@ struct DevBuf {
     return e;
   }
   void release() {
-    if (p) (void)hipFree(p);
+    if (p && !fixed) (void)hipFree(p);
     p = nullptr;
     bytes = 0;
   }
@@ -64,6 +75,9 @@ struct lpm_ctx {
   lpm::DevBuf list;
   lpm::DevBuf tgt, src, partial;
   lpm::DevBuf red;  // reductions
+  // this device's copy of the static scratch array (lpm_api.hip), resolved
+  // once at lpm_create; used by the allocation-free entry point only
+  void* scratch = nullptr;
   // tuning per kind and precision
   lpm::Tuning tune[LPM_K_COUNT][2];
   // timing
@@ -128,6 +142,10 @@ struct PairProblem {
   DevBuf* src_buf = nullptr;
   DevBuf* partial_buf = nullptr;
   DevBuf* red_buf = nullptr;  // FP32 sphere Green: the two FP64 source totals
+  // Source count on the device (set by a device-side compaction): ns is then
+  // an upper bound that sizes the launches, and the kernels read the count.
+  // Such a sum also records no timing events.
+  const int* ns_dev = nullptr;
 };
 
 // Epilogues of the PSE derivative kinds.
@@ -149,6 +167,7 @@ struct PackArgs {
   const double* q3 = nullptr;
   int m = 1;        // derivative fields
   double aux = 0.0; // SWE: log2(e)/eps^2 (FP32) or 1/eps^2 (FP64)
+  const int* count_dev = nullptr;  // sources: record count on the device (count is then a bound)
 };
 
 // Ordered sum over source splits and the FP64 epilogue of one kind.

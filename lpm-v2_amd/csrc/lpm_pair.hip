@@ -9,7 +9,7 @@
 #include <math.h>
 
 #include <algorithm>
-#include <map>
+#include <mutex>
 
 #include "lpm_internal.hpp"
 #include "pair_kernels.hpp"
@@ -44,7 +44,8 @@ __device__ __forceinline__ T idx_word(int p) {
 template <typename T, int KIND>
 __global__ __launch_bounds__(256) void pack_sources(PackArgs a, T* __restrict__ rec) {
   int j = blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= a.count) return;
+  const int count = a.count_dev ? *a.count_dev : a.count;
+  if (j >= count) return;
   const int p = a.list[j];
   T r[SREC];
 #pragma unroll
@@ -338,22 +339,25 @@ static void launch_finalize(int kind, const FinArgs& f, hipStream_t st) {
   }
 }
 
-static std::map<KernelFn, int>& occupancy_cache() {
-  static std::map<KernelFn, int> m;
-  return m;
-}
+// Resident 256-thread blocks per CU of a kernel, cached in a fixed table (no
+// heap allocation, so the allocation-free entry point may use it; a kernel
+// that finds the table full is queried every time).
+struct OccEntry {
+  const void* fn;
+  int nb;
+};
+constexpr int kOccSlots = 1024;
+static OccEntry g_occ[kOccSlots];
+static int g_occ_used = 0;
+static std::mutex g_occ_mutex;
 
 int blocks_per_cu(const void* kfn) {
-  KernelFn fn = reinterpret_cast<KernelFn>(const_cast<void*>(kfn));
-  auto& m = occupancy_cache();
-  auto it = m.find(fn);
-  if (it != m.end()) return it->second;
+  std::lock_guard<std::mutex> lock(g_occ_mutex);
+  for (int k = 0; k < g_occ_used; ++k)
+    if (g_occ[k].fn == kfn) return g_occ[k].nb;
   int nb = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, reinterpret_cast<const void*>(fn), 256, 0) !=
-          hipSuccess ||
-      nb <= 0)
-    nb = 1;
-  m[fn] = nb;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, kfn, 256, 0) != hipSuccess || nb <= 0) nb = 1;
+  if (g_occ_used < kOccSlots) g_occ[g_occ_used++] = {kfn, nb};
   return nb;
 }
 
@@ -474,12 +478,19 @@ int run_pair_sum(lpm_ctx* ctx, const PairProblem& pp) {
   KernelFn fn = ops ? reinterpret_cast<KernelFn>(const_cast<void*>(ops_pick_kernel(kind, prec, tu.r, pp.m, variant)))
                     : pick_kernel(kind, prec, tu.r, variant);
   if (!fn) return fail(ctx, LPM_ERR_ARG, "no pair kernel for this kind, precision and field count");
+  if (pp.ns_dev && (ops || kind == LPM_K_SPHERE_GREEN))
+    return fail(ctx, LPM_ERR_UNSUPPORTED, "a device-side source count is not supported for this kind");
   int nsplit = 1, chunk = std::max(ns, 1);
   if (ns > 0) {
-    const int split_mem = (int)std::min<size_t>(4096, kPartialCap / ((size_t)no * nt * sizeof(double)));
+    const size_t split_bytes = (size_t)no * nt * sizeof(double);
+    size_t cap = std::min<size_t>(4096, kPartialCap / split_bytes);
+    if (pb.fixed) cap = std::min(cap, pb.bytes / split_bytes);
+    if (cap < 1) return fail(ctx, LPM_ERR_NOMEM, "partial buffer too small for one source split");
+    const int split_mem = (int)cap;
     int want = tu.splits > 0 ? tu.splits
                              : choose_splits(n_wtiles, ns, tu.kt, blocks_per_cu((const void*)fn) * ctx->num_cus,
                                              split_mem);
+    if (pb.fixed) want = std::min(want, split_mem);
     chunk = (ns + want - 1) / want;
     chunk = (chunk + tu.kt - 1) / tu.kt * tu.kt;
     nsplit = (ns + chunk - 1) / chunk;
@@ -494,12 +505,15 @@ int run_pair_sum(lpm_ctx* ctx, const PairProblem& pp) {
   pa.q2 = pp.q2;
   pa.q3 = pp.q3;
   pa.m = pp.m;
+  PackArgs pt = pa;  // targets: the count is nt
+  pa.count_dev = pp.ns_dev;
   const bool scaled = kind == LPM_K_PSE8 || kind == LPM_K_PSE2 || kind == LPM_K_PSE_SPHERE8 ||
                       kind == LPM_K_PSE_DERIV_PLANE || kind == LPM_K_PSE_DERIV_SPHERE ||
                       kind == LPM_K_PSE_INTERP_PLANE || kind == LPM_K_PSE_INTERP_SPHERE;
   if (scaled) pa.sigma = (fp32 ? std::sqrt(kLog2e) : 1.0) / pp.eps;
   if (kind == LPM_K_SWE_PLANE) pa.aux = (fp32 ? kLog2e : 1.0) / (pp.eps * pp.eps);
-  PackArgs pt = pa;
+  pt.sigma = pa.sigma;
+  pt.aux = pa.aux;
   if (pp.tx) {
     pt.x = pp.tx;
     pt.y = pp.ty;
@@ -525,10 +539,11 @@ int run_pair_sum(lpm_ctx* ctx, const PairProblem& pp) {
     a.nsplit = nsplit;
     a.kt = tu.kt;
     a.n_wtiles = n_wtiles;
+    a.ns_dev = pp.ns_dev;
     const int nblk_t = (n_wtiles + 3) / 4;
     dim3 grid((unsigned)(nblk_t * nsplit)), block(256);
     hipEvent_t e0 = nullptr, e1 = nullptr;
-    if (ctx->timing) {
+    if (ctx->timing && !pp.ns_dev) {
       e0 = take_event(ctx);
       e1 = take_event(ctx);
       if (e0 && e1) LPM_HIP(ctx, hipEventRecord(e0, ctx->stream));

@@ -461,6 +461,9 @@ struct PairArgs {
   int nsplit;
   int kt;            // sources per compensated tile
   int n_wtiles;      // wave target tiles = ceil(nt / (64 R))
+  // when set, the number of sources is read here and ns is only the bound
+  // that sized chunk * nsplit (splits past the count sum nothing)
+  const int* ns_dev = nullptr;
 };
 
 template <typename T>
@@ -589,8 +592,9 @@ __global__ __launch_bounds__(256) void pair_sum_kernel(PairArgs a) {
   const int split = blockIdx.x / nblk_t;
   const int wtile = (blockIdx.x - split * nblk_t) * 4 + wid;
   if (wtile >= a.n_wtiles) return;  // wave-uniform
+  const int ns = a.ns_dev ? *a.ns_dev : a.ns;
   const int j_begin = split * a.chunk;
-  const int j_end = min(a.ns, j_begin + a.chunk);
+  const int j_end = min(ns, j_begin + a.chunk);
   pair_tile<P, T, R, COMPENSATED>(reinterpret_cast<const Vec4<T>*>(a.tgt), a.nt, wtile * 64 * R,
                                   a.tgt_idx0, reinterpret_cast<const T*>(a.src), j_begin, j_end, a.kt,
                                   a.partial + (size_t)split * P::NOUT * a.nt, a.nt);

@@ -58,6 +58,9 @@ _SIGS = {
     "lpm_last_error": ([_vp], ctypes.c_char_p),
     "lpm_synchronize": ([_vp], _i),
     "lpm_bve_sphere_velocity": ([_vp, _pd, _pd, _pd, _pd, _pd, _pd, _pd, _pd, _d, _d, _pi, _i, _i, _i, _i], _i),
+    # device-memory arrays: passed as addresses
+    "lpm_bve_sphere_velocity_noalloc": ([_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _d, _d, _vp, _i, _i, _i, _i],
+                                        _i),
     "lpm_plane_velocity": ([_vp, _pd, _pd, _pd, _pd, _pd, _pd, _pi, _i, _i, _i, _i], _i),
     "lpm_pse_plane_laplacian": ([_vp, _pd, _pd, _pd, _pd, _pd, _pi, _i, _i, _i, _d, _i, _i], _i),
     "lpm_pse_sphere_laplacian": ([_vp, _pd, _pd, _pd, _pd, _pd, _pd, _pi, _i, _i, _i, _d, _d, _i], _i),
