@@ -86,6 +86,7 @@ def cpu_baseline(m, zeta, target_seconds: float, max_threads: int):
         "sample": f"FP64 oracle BVESphereVelocity loop, targets [0,{b}) x {ns} active sources "
                   f"({pairs / 1e9:.2f} G pairs, {dt:.1f} s)",
         "cpu_model": host_cpu_model(),
+        "host": host_description(),
     }
     # the same loop on one core, over a block sized to about a tenth of the budget
     oracle.set_threads(1)
@@ -98,6 +99,61 @@ def cpu_baseline(m, zeta, target_seconds: float, max_threads: int):
                                                               f"{dt1:.1f} s)"}
     oracle.set_threads(max_threads)
     return out
+
+
+def velocity_error_fp64(ctx, st: dict, m) -> dict:
+    """Normwise max, RMS and pointwise errors of the stored FP32 velocity of a
+    solver state against the FP64 GPU direct sum of the same state."""
+    g64 = np.array(ctx.bve_sphere_velocity(st["x"], st["y"], st["z"], st["relvort"], m.area, m.active,
+                                           precision="fp64"))
+    g32 = np.array([st["u"], st["v"], st["w"]])
+    d = np.sqrt(((g32 - g64) ** 2).sum(0))
+    mag = np.sqrt((g64**2).sum(0))
+    return {
+        "max_rel": float(d.max() / mag.max()),
+        "rms_rel": float(np.sqrt((d**2).sum() / (mag**2).sum())),
+        "pointwise_max_rel": float((d / np.maximum(mag, 1e-3 * mag.max())).max()),
+        "targets": int(d.shape[0]),
+    }
+
+
+def host_description() -> dict:
+    """Host CPUs as the process sees them: affinity set, nproc, model string,
+    family/model numbers (which identify the core generation where a
+    hypervisor shortens the name) and lscpu's model name."""
+    import subprocess
+
+    info = {"affinity_cpus": len(os.sched_getaffinity(0)), "os_cpu_count": os.cpu_count()}
+    try:
+        info["nproc"] = int(subprocess.run(["nproc"], capture_output=True, text=True, timeout=10).stdout.strip())
+    except (OSError, ValueError, subprocess.SubprocessError):
+        pass
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                k, _, v = line.partition(":")
+                k = k.strip()
+                if k in ("model name", "cpu family", "model", "stepping", "cpu MHz") and k not in info:
+                    info[k.replace(" ", "_")] = v.strip()
+                if k == "flags":
+                    break
+    except OSError:
+        pass
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        for line in out.splitlines():
+            k, _, v = line.partition(":")
+            if k.strip() in ("Model name", "Socket(s)", "Core(s) per socket", "Thread(s) per core"):
+                info["lscpu_" + k.strip().lower().replace("(s)", "s").replace(" ", "_")] = v.strip()
+    except (OSError, subprocess.SubprocessError):
+        pass
+    return info
+
+
+def default_cpu_threads() -> int:
+    n = len(os.sched_getaffinity(0))
+    cap = os.environ.get("OMP_NUM_THREADS")
+    return max(1, min(n, int(cap))) if cap and cap.isdigit() else n
 
 
 def host_cpu_model() -> str:
@@ -241,12 +297,17 @@ def pse_ops_cpu_baseline(res: dict, max_threads: int, plane_block: int = 2048, s
         log(f"[cpu {name}] {entry}")
 
 
-def learned_far_field(ctx, m, zeta, omega, dt, plain_ms_per_step, steps, samples, epochs):
-    """The learned far-field form on the same workload (1,280 level-3 cells, near
-    radius 0.15, network band to 0.30, 4th-order cell expansion beyond): train on
-    direct-sum samples of the method's own run, then `steps` RK4 steps with the
-    plain sum judging every 10 steps (fallback above 1e-4) and fine-tuning the
-    network on each judged state.  Returns the speedup over the plain steps."""
+def learned_far_field(ctx, m, zeta, omega, dt, plain_ms_per_step, steps, samples, epochs, plain_end=None,
+                      plain_end_step=0):
+    """The learned far-field form on the same workload, as configured: 1,280
+    level-3 cells, the near field within 0.1 rad summed directly (FP32
+    compensated), every far cell by its 4th-order expansion in its moments, and
+    the network correcting the expansion's remainder for the cells within
+    0.3 rad.  Trained on direct-sum samples of the method's own run, then
+    `steps` RK4 steps with the plain sum judging every 10 steps (fallback above
+    1e-4) and fine-tuning the network on each judged state.  When the plain
+    FP32 run's state after `plain_end_step` steps is given, the learned run's
+    state after as many steps is compared with it (trajectory error)."""
     from lpmgpu import _lib as L
     from lpmgpu.farfield import EXPANSION, LEARNED, PLAIN, FarFieldBVE
 
@@ -254,8 +315,9 @@ def learned_far_field(ctx, m, zeta, omega, dt, plain_ms_per_step, steps, samples
         a, b = np.array(a), np.array(b)
         return float(np.sqrt(((a - b) ** 2).sum(0)).max() / np.sqrt((b**2).sum(0)).max())
 
-    near, band, order = 0.15, 0.30, 4
-    f = FarFieldBVE(ctx, m, zeta, rotation_rate=omega, near_radius=near, band_radius=band, order=order)
+    near, band, order, draws = 0.10, 0.30, 4, 7
+    f = FarFieldBVE(ctx, m, zeta, rotation_rate=omega, near_radius=near, band_radius=band, order=order,
+                    sample_draws=draws)
     ctx.reset_kernel_stats()
     p = f.velocity(PLAIN)
     _, ms_plain = ctx.kernel_stats(L.K_SPHERE_BS)
@@ -272,16 +334,38 @@ def learned_far_field(ctx, m, zeta, omega, dt, plain_ms_per_step, steps, samples
     log(f"[bench] far field: trained in {train_s:.1f}s (loss {loss:.3e}); t=0 error {e_exp:.3e} without the "
         f"network, {e_net:.3e} with it; one evaluation {ms_plain:.1f} ms plain, {ms_net:.1f} ms learned")
     f.set_online(250_000, epochs=1, batch=4096, lr=2e-4)
-    judged, mode = [], LEARNED
-    ctx.synchronize()
-    t0 = time.perf_counter()
-    for k in range(steps // 10):
-        r = f.steps(dt, 10, LEARNED, judge_every=10, tol=1e-4)
-        judged.append(r["last_error"])
+    # segments of the run: judged every 10 steps; when the trajectory is
+    # compared at a step count that is not a multiple of 10, the run stops
+    # there for the snapshot and the next segment ends on the next multiple
+    cut = plain_end_step if plain_end is not None and 0 < plain_end_step <= steps else 0
+    marks = sorted(set(list(range(10, steps + 1, 10)) + ([cut] if cut else [])))
+    judged, mode, done, run_s, traj = [], LEARNED, 0, 0.0, None
+    for mk in marks:
+        n = mk - done
+        ctx.synchronize()
+        t1 = time.perf_counter()
+        r = f.steps(dt, n, mode, judge_every=n if mk % 10 == 0 else 0, tol=1e-4)  # a fallback is for good
+        ctx.synchronize()
+        run_s += time.perf_counter() - t1
+        done = mk
         mode = r["mode"]
-        log(f"[bench] far field: steps {10 * (k + 1)} judged error {r['last_error']:.3e} mode {mode}")
-    ctx.synchronize()
-    ms_step = 1e3 * (time.perf_counter() - t0) / max(1, 10 * (steps // 10))
+        if mk % 10 == 0 and r["judged"] > 0:
+            judged.append(r["last_error"])
+            log(f"[bench] far field: steps {mk} judged error {r['last_error']:.3e} mode {mode}")
+        if mk == cut:
+            st = f.state()
+            traj = {
+                "step": mk,
+                "vs": "plain FP32-compensated RK4 run from the same initial state",
+                "position_max_abs": float(np.abs(np.stack([st["x"], st["y"], st["z"]]) -
+                                                 np.stack([plain_end["x"], plain_end["y"], plain_end["z"]])).max()),
+                "relvort_max_rel": float(np.abs(st["relvort"] - plain_end["relvort"]).max() /
+                                         np.abs(plain_end["relvort"]).max()),
+                "velocity_max_rel": rel((st["u"], st["v"], st["w"]),
+                                        (plain_end["u"], plain_end["v"], plain_end["w"])),
+            }
+            log(f"[bench] far field: trajectory after {mk} steps vs the plain run: {traj}")
+    ms_step = 1e3 * run_s / max(1, done)
     f.delete()
     return {
         "speedup_vs_plain": round(plain_ms_per_step / ms_step, 3),
@@ -289,13 +373,14 @@ def learned_far_field(ctx, m, zeta, omega, dt, plain_ms_per_step, steps, samples
         "err_definition": "max_i |u_i - u_plain_i| / max_i |u_plain_i| at every judged step (plain = FP32 sum)",
         "ms_per_step": round(ms_step, 2),
         "plain_ms_per_step": round(plain_ms_per_step, 2),
-        "steps": 10 * (steps // 10),
+        "steps": done,
         "judge_every": 10,
         "tol": 1e-4,
         "fell_back": mode != LEARNED,
         "judged_errors": judged,
         "includes": "judge sums and online fine-tuning (250k samples, 1 epoch) at every judge",
         "t0_error": {"without_network": e_exp, "with_network": e_net, "gain": round(e_exp / e_net, 3)},
+        "trajectory_error": traj,
         "eval_ms": {"plain": round(ms_plain, 2), "expansion": round(ms_exp, 2), "learned": round(ms_net, 2)},
         "cells": int(f.ncell),
         "near_radius": near,
@@ -304,7 +389,8 @@ def learned_far_field(ctx, m, zeta, omega, dt, plain_ms_per_step, steps, samples
         "near_cells_mean": float(nc.mean()),
         "band_cells_mean": float(bc.mean()),
         "training": {"samples": samples, "epochs": epochs, "seconds": round(train_s, 1), "loss": loss,
-                     "states": "5 states 5 steps apart of an expansion-mode run, then back to t = 0"},
+                     "states": "5 states 5 steps apart of an expansion-mode run, then back to t = 0",
+                     "band_cell_draws": draws},
     }
 
 
@@ -315,9 +401,14 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--level", type=int, default=8)
     ap.add_argument("--dt", type=float, default=0.01)
-    ap.add_argument("--err-block", type=int, default=32768, help="targets checked against FP64 (0 = skip)")
+    ap.add_argument("--full-error", type=int, default=1,
+                    help="FP32 vs FP64 over every target at t = 0 and after the timed steps (0 = skip)")
+    ap.add_argument("--fp64-steps", type=int, default=2, help="steps timed with FP64 pair sums (0 = skip)")
+    ap.add_argument("--stream-steps", type=int, default=2,
+                    help="steps timed with the stream functions in every step (0 = skip)")
     ap.add_argument("--cpu-seconds", type=float, default=10.0, help="CPU baseline budget (0 = skip)")
-    ap.add_argument("--cpu-threads", type=int, default=int(os.environ.get("OMP_NUM_THREADS", "16")))
+    ap.add_argument("--cpu-threads", type=int, default=default_cpu_threads(),
+                    help="oracle threads (default: the CPUs this process may run on, capped by OMP_NUM_THREADS)")
     ap.add_argument("--graph", action="store_true", help="replay steps as hipGraphs (no per-kernel timing)")
     ap.add_argument("--workload", default="n655k",
                     help="n655k (default, the headline line) or cfg0/cfg1/cfg2/cfg3/pse_ops for the other workloads")
@@ -380,26 +471,12 @@ def main():
                               precision="fp32", use_graph=args.graph, project=True)
     log("[bench] solver ready (initial velocity computed)")
 
-    # accuracy of the FP32 sum vs the FP64 GPU sum on a block of targets
+    # accuracy of the stored FP32 velocity against the FP64 GPU sum of the same
+    # state, over every target, before the warm-up and after the timed steps
     err = {}
-    if args.err_block > 0:
-        b = min(args.err_block, T)
-        i0 = m.n_vertices - b // 2  # straddle vertices and centres
-        i1 = i0 + b
-        g32 = np.array(ctx.bve_sphere_velocity(m.x, m.y, m.z, zeta, m.area, m.active, i0=i0, i1=i1,
-                                               precision="fp32"))[:, i0:i1]
-        g64 = np.array(ctx.bve_sphere_velocity(m.x, m.y, m.z, zeta, m.area, m.active, i0=i0, i1=i1,
-                                               precision="fp64"))[:, i0:i1]
-        d = np.sqrt(((g32 - g64) ** 2).sum(0))
-        mag = np.sqrt((g64**2).sum(0))
-        err = {
-            "max_rel": float(d.max() / mag.max()),
-            "rms_rel": float(np.sqrt((d**2).sum() / (mag**2).sum())),
-            "pointwise_max_rel": float((d / np.maximum(mag, 1e-3 * mag.max())).max()),
-            "targets": [int(i0), int(i1)],
-            "vs": "FP64 GPU direct sum",
-        }
-        log(f"[bench] FP32 vs FP64 on {b} targets: {err}")
+    if args.full_error:
+        err["t0"] = velocity_error_fp64(ctx, solver.state(), m)
+        log(f"[bench] FP32 vs FP64 over all {T} targets at t = 0: {err['t0']}")
 
     for _ in range(args.warmup):
         solver.timestep(args.dt)
@@ -435,14 +512,37 @@ def main():
             "launches": launches,
             "flops_per_launch": FLOPS_PER_PAIR * pairs_per_sum,
         }
+    end_state = solver.state()
+    if args.full_error:
+        err["end"] = velocity_error_fp64(ctx, end_state, m)
+        err["end_step"] = args.warmup + args.steps
+        log(f"[bench] FP32 vs FP64 over all targets after {args.warmup + args.steps} steps: {err['end']}")
+        err["vs"] = "FP64 GPU direct sum of the same state, every target"
     ke, ens, circ = solver.diagnostics()
     solver.delete()
+
+    # the same steps at the reference's precision (FP64 pair sums), and with
+    # the stream functions at the end of every step (SphereBVESolver.f90:352)
+    other = {}
+    for key, kw, nsteps in (("fp64", dict(precision="fp64"), args.fp64_steps),
+                            ("with_stream", dict(precision="fp32", stream_each_step=True), args.stream_steps)):
+        if nsteps <= 0:
+            continue
+        s2 = lpmgpu.BVESolver(ctx, m.x, m.y, m.z, zeta, m.area, m.active, absvort=zeta + 2 * omega * m.z,
+                              radius=1.0, rotation_rate=omega, project=True, **kw)
+        ctx.synchronize()
+        t1 = time.perf_counter()
+        s2.timestep(args.dt, nsteps)
+        ctx.synchronize()
+        other[key] = 1e3 * (time.perf_counter() - t1) / nsteps
+        s2.delete()
+        log(f"[bench] {key}: {other[key]:.1f} ms per step over {nsteps} steps")
 
     ff = None
     if args.ff_steps > 0:
         log("[bench] learned far field ...")
         ff = learned_far_field(ctx, m, zeta, omega, args.dt, ms_per_step, args.ff_steps, args.ff_samples,
-                               args.ff_epochs)
+                               args.ff_epochs, plain_end=end_state, plain_end_step=args.warmup + args.steps)
         log(f"[bench] learned far field: {ff['speedup_vs_plain']}x at max error {ff['max_rel_err']}")
 
     cpu = None
@@ -471,8 +571,15 @@ def main():
             "sources": S,
             "pairs_per_step": 4 * pairs_per_sum,
             "dt": args.dt,
+            "step": "one classical RK4 step: 4 Biot-Savart sums (stage 1 reuses the stored velocity) and FP64 "
+                    "stage algebra with projected stages; the stream-function sum the reference ends each step "
+                    "with (SphereBVESolver.f90:352) is not in it (see rk4_steps_per_s_with_stream)",
+            "stream_functions_each_step": False,
         },
         "rk4_steps_per_s": round(1e3 / ms_per_step, 4),
+        "rk4_steps_per_s_with_stream": round(1e3 / other["with_stream"], 4) if "with_stream" in other else None,
+        "fp64_ms_per_step": round(other["fp64"], 1) if "fp64" in other else None,
+        "fp64_rk4_steps_per_s": round(1e3 / other["fp64"], 4) if "fp64" in other else None,
         "velocity_error": err,
         "diagnostics": {"ke": ke, "enstrophy": ens, "circulation": circ},
         "roofline": roofline,

@@ -399,7 +399,10 @@ int lpm_mlp_delete(lpm_mlp *mlp);
  * moments; P = 4: 35), and cells with a < band_radius + rho_t + rho_s also get
  * the network's correction of the expansion remainder (BF16 MFMA).  Stage
  * points are projected onto the sphere (as with LPM_SOLVER_PROJECT_STAGES).
- * flags: LPM_FF_ORDER(P) with P in 2..4 (0 selects 4).  No counterpart in the
+ * flags: LPM_FF_ORDER(P) with P in 2..4 (0 selects 4), plus LPM_FF_SAMPLE_DRAWS(k):
+ * each training sample draws k band cells of its target (k in 1..7, 0 = 1) and
+ * keeps the one nearest the target, so that the near band, whose remainders
+ * dominate the error, is sampled more.  No counterpart in the
  * reference.
  * The network (13 -> 128 -> 128 -> 128 -> 3, ISRU) sees a cell in the cell's
  * frame Q (rows e1, e2, e3: e3 along the centroid c, e1 towards the cell's
@@ -413,6 +416,8 @@ int lpm_mlp_delete(lpm_mlp *mlp);
 typedef struct lpm_bve_ff lpm_bve_ff;
 #define LPM_FF_ORDER_MASK 7
 #define LPM_FF_ORDER(p) ((p) & LPM_FF_ORDER_MASK)
+#define LPM_FF_SAMPLE_DRAWS_MASK 0x38
+#define LPM_FF_SAMPLE_DRAWS(k) (((k) & 7) << 3)
 
 int lpm_bve_ff_new(lpm_ctx *ctx, lpm_bve_ff **ff, int n, const double *x, const double *y,
                    const double *z, const double *relvort, const double *area,

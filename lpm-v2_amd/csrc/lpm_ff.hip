@@ -868,6 +868,7 @@ struct SampleArgs {
   float* X;  // [n][13]
   float* Y;  // [n][3]
   int* ids;  // [n][2]: target particle, source cell (-1, -1 for an invalid sample)
+  int draws = 1;  // band cells drawn per sample; the one nearest the target is kept
 };
 
 __device__ __forceinline__ unsigned long long mix64(unsigned long long z) {
@@ -898,10 +899,22 @@ __global__ __launch_bounds__(256) void ff_make_samples(SampleArgs a) {
     const int t = a.tgt_cell[k];
     const int nb = a.band_count[t];
     if (nb == 0) continue;
-    const int s = a.band_list[(size_t)t * a.ncell + (int)(h % (unsigned long long)nb)];
-    h = mix64(h);
     const int p = a.tgt_perm[k];
     const double x = a.x[p], y = a.y[p], z = a.z[p];
+    // band cell: the nearest (centroid distance) of `draws` uniform draws, so
+    // that the cells whose remainders dominate the error are sampled more
+    int s = -1;
+    double best = 0.0;
+    for (int d = 0; d < a.draws; ++d) {
+      const int c = a.band_list[(size_t)t * a.ncell + (int)(h % (unsigned long long)nb)];
+      h = mix64(h);
+      const double* cc = a.cen64 + (size_t)c * 3;
+      const double d2 = (x - cc[0]) * (x - cc[0]) + (y - cc[1]) * (y - cc[1]) + (z - cc[2]) * (z - cc[2]);
+      if (s < 0 || d2 < best) {
+        s = c;
+        best = d2;
+      }
+    }
     const double R = a.radius;
     double Ge[3] = {0.0, 0.0, 0.0};
     for (int j = a.src_start[s]; j < a.src_start[s + 1]; ++j) {
@@ -1030,6 +1043,7 @@ struct lpm_bve_ff {
   int n = 0, ns = 0, ncell = 0;
   double radius = 1.0, omega = 0.0, r_near = 0.1, r_band = 0.45;
   int order = 4;  // expansion order of far cells
+  int draws = 1;  // training samples: band cells drawn per sample (nearest kept)
   DevBuf x, y, z, q, u, v, w, area;
   DevBuf k[4][4];
   DevBuf xi, yi, zi, qi;
@@ -1337,8 +1351,10 @@ int lpm_bve_ff_new(lpm_ctx* ctx, lpm_bve_ff** out, int n, const double* x, const
   if (!(radius > 0.0) || !(near_radius >= 0.0) || !(band_radius >= near_radius))
     return fail(ctx, LPM_ERR_ARG, "lpm_bve_ff_new: need radius > 0 and 0 <= near_radius <= band_radius");
   const int order = (flags & LPM_FF_ORDER_MASK) == 0 ? 4 : (flags & LPM_FF_ORDER_MASK);
-  if (order < 2 || order > 4 || (flags & ~LPM_FF_ORDER_MASK))
-    return fail(ctx, LPM_ERR_ARG, "lpm_bve_ff_new: flags hold the expansion order 2, 3 or 4 (0: 4) only");
+  if (order < 2 || order > 4 || (flags & ~(LPM_FF_ORDER_MASK | LPM_FF_SAMPLE_DRAWS_MASK)))
+    return fail(ctx, LPM_ERR_ARG,
+                "lpm_bve_ff_new: flags hold the expansion order 2, 3 or 4 (0: 4) and the sample draws only");
+  const int draws = std::max(1, (flags & LPM_FF_SAMPLE_DRAWS_MASK) >> 3);
   for (int i = 0; i < n; ++i)
     if (cell[i] < 0 || cell[i] >= ncell) return fail(ctx, LPM_ERR_ARG, "lpm_bve_ff_new: cell id out of range");
   LPM_HIP(ctx, hipSetDevice(ctx->device));
@@ -1351,6 +1367,7 @@ int lpm_bve_ff_new(lpm_ctx* ctx, lpm_bve_ff** out, int n, const double* x, const
   f->r_near = near_radius;
   f->r_band = band_radius;
   f->order = order;
+  f->draws = draws;
   // Cells are fixed regions of the sphere: each is the set of points nearest to
   // its generator, the direction of the area-weighted centroid of the given
   // cell's active sources at set-up (of all its particles if it has none).  Its
@@ -1470,6 +1487,7 @@ int lpm_bve_ff_collect(lpm_bve_ff* f, int n_samples, uint64_t seed, int64_t* tot
                 f->cen64.as<double>(), f->mom64.as<double>(), f->mlp_rec.as<float>(), f->radius,
                 f->sx.as<float>() + (size_t)f->n_collected * 13, f->sy.as<float>() + (size_t)f->n_collected * 3,
                 f->sid.as<int>() + (size_t)f->n_collected * 2};
+  sa.draws = f->draws;
   hipLaunchKernelGGL(ff_make_samples, dim3((n_samples + 255) / 256), dim3(256), 0, ctx->stream, sa);
   LPM_HIP(ctx, hipGetLastError());
   LPM_HIP(ctx, hipStreamSynchronize(ctx->stream));

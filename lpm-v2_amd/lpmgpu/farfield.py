@@ -41,7 +41,7 @@ class FarFieldBVE:
     their moments, with a network correcting the remainder in the band."""
 
     def __init__(self, ctx, mesh, relvort, rotation_rate=0.0, near_radius=0.1, band_radius=0.45, radius=1.0,
-                 order=4):
+                 order=4, sample_draws=1):
         if mesh.face_cell is None or mesh.cell_level < 0:
             raise ValueError("mesh needs coarse cells (cell_level >= 0)")
         self.ctx = ctx
@@ -54,7 +54,7 @@ class FarFieldBVE:
         h = ctypes.c_void_p()
         ctx.check(ctx._lib.lpm_bve_ff_new(ctx.handle, ctypes.byref(h), self.n, *[pd(a) for a in arr], pi(act),
                                           pi(self.cell), self.ncell, radius, rotation_rate, near_radius, band_radius,
-                                          order))
+                                          order | ((sample_draws & 7) << 3)))
         self.handle = h
         ctx.adopt(self)
 

@@ -13,9 +13,12 @@ pytestmark = pytest.mark.gpu
 
 GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 # FP64: different summation order and denominator form than the reference's
-# loop; FP32 compensated: the workloads' bars (velocity 1e-6, stream 2e-6, PSE 1e-4)
+# loop.  FP32 compensated: the pair terms are rounded to FP32, so the error is
+# about 6e-8 times sum|terms| / |u|; on these coarse meshes that ratio is
+# larger than at the workloads' sizes (the RH4 sum on cubed L3 cancels to
+# 1.0e-6, against 3.4e-7 on cubed L7), so velocities are held to 2e-6 here.
 TOL = {"fp64": dict(vel=1e-11, psi=1e-11, pse=1e-11, step=1e-11),
-       "fp32": dict(vel=1e-6, psi=2e-6, pse=1e-4, step=1e-6)}
+       "fp32": dict(vel=2e-6, psi=2e-6, pse=1e-4, step=2e-6)}
 
 
 def load(case):
