@@ -20,7 +20,28 @@ MODS     := TypeDefs Utilities STDIntVector OutputWriter Logger SphereGeometry P
             PlaneIncompressibleSolver PSEDirectSum
 OBJS     := $(addprefix $(OUT)/,$(addsuffix .o,$(MODS)))
 
-all: $(OUT)/ref_golden
+all: $(OUT)/ref_golden $(OUT)/ref_gpu_timestep
+
+# The GPU-backed SphereBVESolverModule (lpm-v2_amd/fortran/SphereBVESolver_lpm.f90)
+# linked with the reference's other modules in place of its SphereBVESolver,
+# driven by the reference's own call sequence (tests/golden/ref_gpu_timestep.f90).
+# Its .mod files go to a directory of their own, searched first, since the
+# module name is the reference's.
+LPMLIB   ?= lpm-v2_amd
+GPUMOD   := $(OUT)/mod_gpu
+GPUOBJS  := $(filter-out $(OUT)/SphereBVESolver.o,$(OBJS))
+
+$(OUT)/ref_gpu_timestep: tests/golden/ref_gpu_timestep.f90 lpm-v2_amd/fortran/SphereBVESolver_lpm.f90 \
+                         lpm-v2_amd/fortran/lpm_direct_f.f90 $(OUT)/lpm_ref_modules.stamp
+	mkdir -p $(GPUMOD)
+	$(AMDFLANG) -cpp -O2 -J $(GPUMOD) -c lpm-v2_amd/fortran/lpm_direct_f.f90 -o $(GPUMOD)/lpm_direct_f.o
+	$(AMDFLANG) -cpp -O2 -I$(MPI)/include -J $(GPUMOD) -I $(OUT)/mod -c lpm-v2_amd/fortran/SphereBVESolver_lpm.f90 \
+		-o $(GPUMOD)/SphereBVESolver_lpm.o
+	$(AMDFLANG) -cpp -O2 -I$(MPI)/include -I $(GPUMOD) -I $(OUT)/mod -J $(GPUMOD) -c tests/golden/ref_gpu_timestep.f90 \
+		-o $(GPUMOD)/ref_gpu_timestep.o
+	$(AMDFLANG) -o $@ $(GPUMOD)/ref_gpu_timestep.o $(GPUMOD)/SphereBVESolver_lpm.o $(GPUMOD)/lpm_direct_f.o \
+		$(GPUOBJS) -L$(MPI)/lib -lmpifort -lmpi -L$(LPMLIB) -llpm_direct -Wl,-rpath,$(MPI)/lib \
+		-Wl,-rpath,'$$ORIGIN/../../$(LPMLIB)'
 
 $(OUT)/mod:
 	mkdir -p $(OUT)/mod
