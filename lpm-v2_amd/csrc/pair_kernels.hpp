@@ -527,12 +527,20 @@ __device__ __forceinline__ void pair_tile(const Vec4<T>* __restrict__ tg, int nt
       for (int k = 0; k < NO; ++k) p[r][k] = (T)0;
 
     if (!checked) {
+      // the next source's record is loaded before this one's arithmetic, so
+      // the scalar loads' latency overlaps the pair work (the last iteration
+      // reloads its own record).  Measured on MI355X against loading each
+      // record at its use: PSE-8 at 512^2 15.96 -> 14.45 ms, plane gradient
+      // 20.68 -> 18.92 ms, sphere Biot-Savart at icosahedral L7 38.17 -> 37.95 ms.
+      T sv[P::NW];
+#pragma unroll
+      for (int q = 0; q < P::NW; ++q) sv[q] = src[(size_t)jt * P::NW + q];
 #pragma unroll 2
       for (int j = jt; j < jt_end; ++j) {
-        const T* s = src + (size_t)j * P::NW;
-        T sv[P::NW];
+        const int jn = j + 1 < jt_end ? j + 1 : j;
+        T sn[P::NW];
 #pragma unroll
-        for (int q = 0; q < P::NW; ++q) sv[q] = s[q];
+        for (int q = 0; q < P::NW; ++q) sn[q] = src[(size_t)jn * P::NW + q];
         W4<T> w[R];
         T f[R];
 #pragma unroll
@@ -541,6 +549,8 @@ __device__ __forceinline__ void pair_tile(const Vec4<T>* __restrict__ tg, int nt
         for (int r = 0; r < R; ++r) f[r] = P::trans(w[r]);
 #pragma unroll
         for (int r = 0; r < R; ++r) P::acc(sv, w[r], f[r], p[r]);
+#pragma unroll
+        for (int q = 0; q < P::NW; ++q) sv[q] = sn[q];
       }
     } else {
       for (int j = jt; j < jt_end; ++j) {
