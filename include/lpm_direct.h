@@ -364,7 +364,7 @@ int lpm_bve_batch_delete(lpm_bve_batch *batch);
 
 /* MLP nin -> width (x nhidden, activation tanh or ISRU) -> nout, parameters per
  * layer W[out][in] (row-major) then b[out] in FP32.  Training runs in FP32 on the GPU
- * (hand-written GEMM forward/backward and Adam); inference runs in BF16 on
+ * (hand-written GEMM forward/backward and Adam); inference runs in FP16 on
  * MFMA with FP32 accumulation (nin <= 32, width 128, nhidden 3, nout <= 4).
  * No counterpart in the reference. */
 typedef struct lpm_mlp lpm_mlp;
@@ -376,7 +376,8 @@ int lpm_mlp_new(lpm_ctx *ctx, lpm_mlp **mlp, int nin, int width, int nhidden, in
 int lpm_mlp_param_count(lpm_mlp *mlp, int64_t *count);
 int lpm_mlp_set_params(lpm_mlp *mlp, const float *params, int64_t count);
 int lpm_mlp_get_params(lpm_mlp *mlp, float *params, int64_t count);
-/* in: [n][nin], out: [n][nout]; use_mfma selects the BF16 MFMA kernel. */
+/* in: [n][nin], out: [n][nout]; use_mfma selects the FP16 MFMA kernel
+ * (inputs, weights and activations rounded to FP16, |values| <= 65504). */
 int lpm_mlp_forward(lpm_mlp *mlp, int n, const float *in, float *out, int use_mfma);
 /* Mean-squared error over n*nout outputs and its gradient (FP32). */
 int lpm_mlp_loss_grad(lpm_mlp *mlp, int n, const float *in, const float *target, float *grad,
@@ -397,7 +398,7 @@ int lpm_mlp_delete(lpm_mlp *mlp);
  * rho_s is summed directly (FP32 compensated), everything else by the cell's
  * expansion of order P in its moments M0..MP about the centroid (P = 2: 10
  * moments; P = 4: 35), and cells with a < band_radius + rho_t + rho_s also get
- * the network's correction of the expansion remainder (BF16 MFMA).  Stage
+ * the network's correction of the expansion remainder (FP16 MFMA).  Stage
  * points are projected onto the sphere (as with LPM_SOLVER_PROJECT_STAGES).
  * flags: LPM_FF_ORDER(P) with P in 2..4 (0 selects 4), plus LPM_FF_SAMPLE_DRAWS(k):
  * each training sample draws k band cells of its target (k in 1..7, 0 = 1) and

@@ -7,7 +7,7 @@
 //   far    otherwise                  : analytic expansion of s to order P (2..4)
 //          in its moments M0..MP of w = zeta A about c_s (P = 4: 35 numbers)
 //   band   far and a < r_band + rho_t + rho_s : far + the MLP's correction of the
-//          expansion's remainder (BF16 MFMA), trained on direct sums
+//          expansion's remainder (FP16 MFMA), trained on direct sums
 // Cell membership is Lagrangian (fixed at set-up); centroids, radii and
 // moments follow the particles and are recomputed at every evaluation.
 //
@@ -755,7 +755,7 @@ struct MlpCorrArgs {
   const int2* items;  // (target cell, 64-target tile)
   const int* n_items;  // device count
   int ncell, nt, order;
-  const __bf16* frag;
+  const lpm::mlp_t* frag;
   const float* bias;
   double* out;        // [3][nt] accumulated into
 };
@@ -763,10 +763,10 @@ struct MlpCorrArgs {
 // 512 threads: 8 waves (2 per SIMD) share one LDS copy of the weights, which
 // alone takes half a CU's LDS.
 __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(2, 2))) void ff_mlp_kernel(MlpCorrArgs a) {
-  __shared__ __attribute__((aligned(16))) __bf16 sfrag[lpm::kMlpFragElems];
+  __shared__ __attribute__((aligned(16))) lpm::mlp_t sfrag[lpm::kMlpFragElems];
   __shared__ __attribute__((aligned(16))) float sbias[lpm::kMlpBiasElems];
   for (int t = threadIdx.x; t < lpm::kMlpFragElems / 8; t += blockDim.x)
-    reinterpret_cast<lpm::bf16x8_t*>(sfrag)[t] = reinterpret_cast<const lpm::bf16x8_t*>(a.frag)[t];
+    reinterpret_cast<lpm::mlpx8_t*>(sfrag)[t] = reinterpret_cast<const lpm::mlpx8_t*>(a.frag)[t];
   for (int t = threadIdx.x; t < lpm::kMlpBiasElems; t += blockDim.x) sbias[t] = a.bias[t];
   __syncthreads();
   constexpr int NCT = kMlpNct;
@@ -801,7 +801,7 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
       float mv[10];
 #pragma unroll
       for (int k = 0; k < 10; ++k) mv[k] = mr[16 + k];
-      lpm::bf16x8_t in8[NCT];
+      lpm::mlpx8_t in8[NCT];
       float loc[NCT][3];
 #pragma unroll
       for (int ct = 0; ct < NCT; ++ct) {
@@ -810,7 +810,7 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
         for (int e = 0; e < 8; ++e) {
           const float f0 = e < 3 ? loc[ct][e] : mv[e - 3];
           const float f1 = e < 5 ? mv[5 + e] : 0.f;
-          in8[ct][e] = (__bf16)(g == 0 ? f0 : (g == 1 ? f1 : 0.f));
+          in8[ct][e] = (lpm::mlp_t)(g == 0 ? f0 : (g == 1 ? f1 : 0.f));
         }
       }
       float o[NCT][4];
@@ -1170,7 +1170,7 @@ int ff_eval(lpm_bve_ff* f, int mode, const double* x, const double* y, const dou
     MlpCorrArgs ma{f->trec.as<lpm::Vec4<float>>(), f->tgt_start.as<int>(), f->mlp_rec.as<float>(),
                    f->band_list.as<int>(), f->band_count.as<int>(), f->mlp_items.as<int2>(), f->item_count.as<int>() + 1, nc, f->n,
                    f->order,
-                   f->net.frag.as<__bf16>(), f->net.bias.as<float>(), f->V.as<double>()};
+                   f->net.frag.as<lpm::mlp_t>(), f->net.bias.as<float>(), f->V.as<double>()};
     hipLaunchKernelGGL(ff_mlp_kernel, dim3(ctx->num_cus), dim3(512), 0, ctx->stream, ma);
   }
   if (ctx->timing && e0 && e1) {
@@ -1261,13 +1261,15 @@ int ff_load(lpm_bve_ff* f, int n, const double* x, const double* y, const double
   }
   const int end_bit = 32 - __builtin_clz((unsigned)ncell);  // keys 0..ncell
   size_t sort_bytes = 0;
-  hipcub::DeviceRadixSort::SortPairs(nullptr, sort_bytes, (const int*)nullptr, (int*)nullptr, (const int*)nullptr,
-                                     (int*)nullptr, n, 0, end_bit, ctx->stream);
+  const hipError_t sort_query =
+      hipcub::DeviceRadixSort::SortPairs(nullptr, sort_bytes, (const int*)nullptr, (int*)nullptr, (const int*)nullptr,
+                                         (int*)nullptr, n, 0, end_bit, ctx->stream);
   f->end_bit = end_bit;
   f->sort_bytes = sort_bytes;
   const size_t nb = (size_t)n * sizeof(double);
   const size_t ni = (size_t)n * sizeof(int);
-  int rc = LPM_OK;
+  int rc = sort_query == hipSuccess ? LPM_OK
+                                    : fail(ctx, LPM_ERR_HIP, "lpm_bve_ff_new: radix-sort workspace query failed");
   auto chk = [&](int r) {
     if (r && !rc) rc = r;
     return rc == LPM_OK;

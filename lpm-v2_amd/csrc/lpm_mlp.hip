@@ -1,5 +1,5 @@
 // lpm_mlp.hip -- far-field surrogate network: parameters, FP32 training
-// kernels (forward, backward, Adam) and the BF16-MFMA inference forward.
+// kernels (forward, backward, Adam) and the FP16-MFMA inference forward.
 #include <math.h>
 #include <string.h>
 
@@ -113,15 +113,15 @@ __global__ __launch_bounds__(256) void sumsq_partial(int64_t n, const float* __r
   if (threadIdx.x == 0) part[blockIdx.x] = sh[0];
 }
 
-// Standalone BF16 inference: features [n][nin] FP32 -> out [n][nout].
+// Standalone FP16 inference: features [n][nin] FP32 -> out [n][nout].
 template <int ACT>
 __global__ __launch_bounds__(256) void mlp_mfma_forward_kernel(int n, int nin, int nout, const float* __restrict__ X,
-                                                               const __bf16* __restrict__ frag,
+                                                               const mlp_t* __restrict__ frag,
                                                                const float* __restrict__ bias, float* __restrict__ Y) {
-  __shared__ __attribute__((aligned(16))) __bf16 sfrag[kMlpFragElems];
+  __shared__ __attribute__((aligned(16))) mlp_t sfrag[kMlpFragElems];
   __shared__ __attribute__((aligned(16))) float sbias[kMlpBiasElems];
   for (int t = threadIdx.x; t < kMlpFragElems / 8; t += blockDim.x)
-    reinterpret_cast<bf16x8_t*>(sfrag)[t] = reinterpret_cast<const bf16x8_t*>(frag)[t];
+    reinterpret_cast<mlpx8_t*>(sfrag)[t] = reinterpret_cast<const mlpx8_t*>(frag)[t];
   for (int t = threadIdx.x; t < kMlpBiasElems; t += blockDim.x) sbias[t] = bias[t];
   __syncthreads();
   constexpr int NCT = 4;  // 64 samples per wave and pass
@@ -129,14 +129,14 @@ __global__ __launch_bounds__(256) void mlp_mfma_forward_kernel(int n, int nin, i
   const int waves = gridDim.x * (blockDim.x >> 6);
   const int wid = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
   for (int base = wid * 16 * NCT; base < n; base += waves * 16 * NCT) {
-    bf16x8_t in8[NCT];
+    mlpx8_t in8[NCT];
 #pragma unroll
     for (int ct = 0; ct < NCT; ++ct) {
       const int s = base + 16 * ct + (lane & 15);
 #pragma unroll
       for (int e = 0; e < 8; ++e) {
         const int f = 8 * g + e;
-        in8[ct][e] = (__bf16)((s < n && f < nin) ? X[(size_t)s * nin + f] : 0.f);
+        in8[ct][e] = (mlp_t)((s < n && f < nin) ? X[(size_t)s * nin + f] : 0.f);
       }
     }
     float out[NCT][4];
@@ -228,7 +228,7 @@ bool Mlp::mfma_ok() const { return nin <= 32 && width == 128 && nhidden == 3 && 
 int Mlp::pack() {
   if (!mfma_ok()) return LPM_OK;
   hipLaunchKernelGGL(mlp_pack_fragments, dim3(64), dim3(256), 0, ctx->stream, p.as<float>(), nin, nout,
-                     frag.as<__bf16>(), bias.as<float>());
+                     frag.as<mlp_t>(), bias.as<float>(), mlp_hidden_scale(act_kind));
   LPM_HIP(ctx, hipGetLastError());
   return LPM_OK;
 }
@@ -420,10 +420,10 @@ int Mlp::forward_mfma(int n, const float* X, float* Y) {
   const int blocks = std::max(1, std::min(2 * ctx->num_cus, (n + 255) / 256));
   if (act_kind == kActIsru)
     hipLaunchKernelGGL(mlp_mfma_forward_kernel<kActIsru>, dim3(blocks), dim3(256), 0, ctx->stream, n, nin, nout, X,
-                       frag.as<__bf16>(), bias.as<float>(), Y);
+                       frag.as<mlp_t>(), bias.as<float>(), Y);
   else
     hipLaunchKernelGGL(mlp_mfma_forward_kernel<kActTanh>, dim3(blocks), dim3(256), 0, ctx->stream, n, nin, nout, X,
-                       frag.as<__bf16>(), bias.as<float>(), Y);
+                       frag.as<mlp_t>(), bias.as<float>(), Y);
   LPM_HIP(ctx, hipGetLastError());
   return LPM_OK;
 }

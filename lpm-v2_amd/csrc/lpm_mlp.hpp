@@ -19,14 +19,14 @@ struct Mlp {
   int adam_t = 0;
   int cur_n = 0;
   DevBuf p, grad, m, v;    // FP32 parameters, gradient, Adam moments
-  DevBuf frag, bias;       // BF16 MFMA fragments + FP32 biases (inference)
+  DevBuf frag, bias;       // FP16 MFMA fragments + FP32 biases (inference)
   DevBuf act, pre, scratch, loss_part, splitk;
   DevBuf xb, tb, idx;      // training minibatch staging
   ~Mlp();
   void release();
   int init(lpm_ctx* c, int nin, int width, int nhidden, int nout, int act, uint64_t seed);
   bool mfma_ok() const;
-  int pack();  // FP32 params -> BF16 fragments
+  int pack();  // FP32 params -> FP16 fragments
   int ensure_batch(int n);
   int forward_f32(int n, const float* X);  // device X [n][nin]; output in act slot nhidden
   int loss_grad(int n, const float* X, const float* T, double* loss);

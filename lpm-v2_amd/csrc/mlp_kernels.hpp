@@ -6,11 +6,11 @@
 //
 // Training path (FP32, VALU): tiled GEMMs C = A op(B) with a fused
 // bias + activation epilogue, the backward GEMMs, column sums and Adam.
-// Inference path (BF16 MFMA, FP32 accumulate), specialised for the
+// Inference path (FP16 MFMA, FP32 accumulate), specialised for the
 // 13 -> 128 -> 128 -> 128 -> 3 surrogate: activations are kept transposed
 // (features = MFMA rows, samples = MFMA columns), so the D registers of one
 // layer (lane l: rows 4(l/16)+v of each 16-row tile, column l%16) are, after
-// bias + activation + BF16 conversion, exactly the B operand of the next layer once
+// bias + activation + FP16 conversion, exactly the B operand of the next layer once
 // the next layer's weight fragments are stored with the matching K order:
 // k-slot e of lane group g in k-step ks is feature 32 ks + 16 (e/4) + 4 g + e%4.
 #pragma once
@@ -20,8 +20,17 @@
 
 namespace lpm {
 
-typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
+// Inference operands are FP16: the same MFMA rate as BF16 on gfx950 (measured:
+// equal launch times) with 3 more mantissa bits, which cut the forward pass's
+// error against FP64 from 4.5e-3 to 6e-4 (relative RMS).
+typedef _Float16 mlp_t;
+typedef mlp_t mlpx8_t __attribute__((ext_vector_type(8)));
 typedef float floatx4_t __attribute__((ext_vector_type(4)));
+typedef _Float16 half2_t __attribute__((ext_vector_type(2)));
+
+__device__ __forceinline__ floatx4_t mfma16(mlpx8_t a, mlpx8_t b, floatx4_t c) {
+  return __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b, c, 0, 0, 0);
+}
 
 __device__ __forceinline__ float tanh_fast(float x) {
   // tanh x = 1 - 2 / (exp(2x) + 1), exp via v_exp_f32 (2^y); saturates cleanly
@@ -40,6 +49,44 @@ __device__ __forceinline__ float act_fast(float x) {
     return x * __builtin_amdgcn_rsqf(fmaf(x, x, 1.0f));
   else
     return tanh_fast(x);
+}
+
+// Scale of the hidden layers' pre-activations in the MFMA path: with ISRU the
+// hidden weights and biases are stored times kIsruScale, so the accumulators
+// hold y = x kIsruScale and ISRU(x) = y / sqrt(y^2 + kIsruScale^2), evaluated
+// in packed FP16 (v_pk_fma_f16, v_rsq_f16, v_pk_mul_f16) without overflow of
+// y^2 for |x| < 16,000.
+constexpr float kIsruScale = 1.0f / 64.0f;
+
+// Activation of 8 accumulator values (lo: one 16-row tile's 4, hi: the next
+// tile's 4) into the next layer's 8-element B operand.
+template <int ACT>
+__device__ __forceinline__ mlpx8_t act8(floatx4_t lo, floatx4_t hi) {
+  mlpx8_t v;
+  if constexpr (ACT == kActIsru) {
+    const _Float16 c = (_Float16)(kIsruScale * kIsruScale);
+#pragma unroll
+    for (int p = 0; p < 4; ++p) {
+      const float a = p < 2 ? lo[2 * p] : hi[2 * p - 4], b = p < 2 ? lo[2 * p + 1] : hi[2 * p - 3];
+      const half2_t y = {(_Float16)a, (_Float16)b};
+      const half2_t t = y * y + (half2_t){c, c};
+      half2_t r;
+      r[0] = __builtin_amdgcn_rsqh(t[0]);
+      r[1] = __builtin_amdgcn_rsqh(t[1]);
+      const half2_t h = y * r;
+      v[2 * p] = h[0];
+      v[2 * p + 1] = h[1];
+    }
+    return v;
+  }
+#pragma unroll
+  for (int e = 0; e < 8; ++e) v[e] = (mlp_t)act_fast<ACT>(e < 4 ? lo[e] : hi[e - 4]);
+  return v;
+}
+
+// Scale the MFMA path applies to the hidden layers' weights and biases.
+__host__ __device__ inline float mlp_hidden_scale(int act) {
+  return act == kActIsru ? kIsruScale : 1.0f;
 }
 
 // ------------------------------------------------------------ FP32 GEMM
@@ -143,7 +190,7 @@ static __global__ __launch_bounds__(256) void gemm_splitk_reduce(int splits, int
   C[(size_t)(t / N) * ldc + t % N] = s;
 }
 
-// ------------------------------------------------------- BF16 MFMA forward
+// ------------------------------------------------------- FP16 MFMA forward
 // Weight fragments, staged once per block in LDS:
 //   W1: [8 rt][64 lanes][8]      (K = 32, inputs 0..nin-1, zero padded)
 //   W2, W3: [8 rt][4 ks][64][8]  (K order permuted as described above)
@@ -152,7 +199,7 @@ static __global__ __launch_bounds__(256) void gemm_splitk_reduce(int splits, int
 constexpr int kW1Frag = 8 * 64 * 8;
 constexpr int kWHFrag = 8 * 4 * 64 * 8;
 constexpr int kW4Frag = 4 * 64 * 8;
-constexpr int kMlpFragElems = kW1Frag + 2 * kWHFrag + kW4Frag;  // bf16 elements
+constexpr int kMlpFragElems = kW1Frag + 2 * kWHFrag + kW4Frag;  // FP16 elements
 constexpr int kMlpBiasElems = 128 * 3 + 16;
 
 __host__ __device__ inline int hidden_k_feature(int ks, int g, int e) {
@@ -161,8 +208,8 @@ __host__ __device__ inline int hidden_k_feature(int ks, int g, int e) {
 
 // Convert FP32 parameters (13 -> 128 x3 -> 3 layout) into fragment order.
 static __global__ __launch_bounds__(256) void mlp_pack_fragments(const float* __restrict__ P, int nin, int nout,
-                                                         __bf16* __restrict__ frag,
-                                                         float* __restrict__ bias) {
+                                                         mlp_t* __restrict__ frag,
+                                                         float* __restrict__ bias, float hscale) {
   const int W = 128;
   const float* W1 = P;
   const float* b1 = W1 + W * nin;
@@ -189,13 +236,13 @@ static __global__ __launch_bounds__(256) void mlp_pack_fragments(const float* __
       const int row = l % 16;
       v = row < nout ? W4[row * W + hidden_k_feature(ks, l / 16, e)] : 0.f;
     }
-    frag[t] = (__bf16)v;
+    frag[t] = (mlp_t)(t < kW1Frag + 2 * kWHFrag ? v * hscale : v);
   }
   for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < kMlpBiasElems; t += gridDim.x * blockDim.x) {
     float v;
-    if (t < 128) v = b1[t];
-    else if (t < 256) v = b2[t - 128];
-    else if (t < 384) v = b3[t - 256];
+    if (t < 128) v = b1[t] * hscale;
+    else if (t < 256) v = b2[t - 128] * hscale;
+    else if (t < 384) v = b3[t - 256] * hscale;
     else v = (t - 384) < nout ? b4[t - 384] : 0.f;
     bias[t] = v;
   }
@@ -207,16 +254,16 @@ static __global__ __launch_bounds__(256) void mlp_pack_fragments(const float* __
 // (lanes 16..63 hold padding rows).  Weights/biases in LDS (fragment order);
 // sbias must be 16-byte aligned (it is read as float4 rows).
 template <int NCT, int ACT = kActTanh>
-__device__ __forceinline__ void mlp_wave_forward(const bf16x8_t in8[NCT], const __bf16* __restrict__ sfrag,
+__device__ __forceinline__ void mlp_wave_forward(const mlpx8_t in8[NCT], const mlp_t* __restrict__ sfrag,
                                                  const float* __restrict__ sbias, float out[NCT][4]) {
   const int lane = threadIdx.x & 63;
   const int g = lane >> 4;
-  const bf16x8_t* F1 = reinterpret_cast<const bf16x8_t*>(sfrag);
-  const bf16x8_t* F2 = reinterpret_cast<const bf16x8_t*>(sfrag + kW1Frag);
-  const bf16x8_t* F3 = reinterpret_cast<const bf16x8_t*>(sfrag + kW1Frag + kWHFrag);
-  const bf16x8_t* F4 = reinterpret_cast<const bf16x8_t*>(sfrag + kW1Frag + 2 * kWHFrag);
+  const mlpx8_t* F1 = reinterpret_cast<const mlpx8_t*>(sfrag);
+  const mlpx8_t* F2 = reinterpret_cast<const mlpx8_t*>(sfrag + kW1Frag);
+  const mlpx8_t* F3 = reinterpret_cast<const mlpx8_t*>(sfrag + kW1Frag + kWHFrag);
+  const mlpx8_t* F4 = reinterpret_cast<const mlpx8_t*>(sfrag + kW1Frag + 2 * kWHFrag);
   floatx4_t acc[8][NCT];
-  bf16x8_t h[4][NCT];  // B operands of the next layer: [ks][ct]
+  mlpx8_t h[4][NCT];  // B operands of the next layer: [ks][ct]
 
   // The accumulators start at the bias (lane group g holds rows 4 g .. 4 g + 3
   // of each 16-row tile), so the MFMA adds it and the activation pass needs no
@@ -227,31 +274,23 @@ __device__ __forceinline__ void mlp_wave_forward(const bf16x8_t in8[NCT], const 
   // layer 1: K = 32 (one step)
 #pragma unroll
   for (int rt = 0; rt < 8; ++rt) {
-    const bf16x8_t a = F1[rt * 64 + lane];
+    const mlpx8_t a = F1[rt * 64 + lane];
     const floatx4_t z = bias4(sbias, rt);
 #pragma unroll
-    for (int ct = 0; ct < NCT; ++ct) acc[rt][ct] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, in8[ct], z, 0, 0, 0);
+    for (int ct = 0; ct < NCT; ++ct) acc[rt][ct] = mfma16(a, in8[ct], z);
   }
   // activation -> next B operands
   auto activate = [&]() {
 #pragma unroll
     for (int ks = 0; ks < 4; ++ks)
 #pragma unroll
-      for (int ct = 0; ct < NCT; ++ct) {
-        bf16x8_t v;
-#pragma unroll
-        for (int e = 0; e < 8; ++e) {
-          const int rt = 2 * ks + e / 4, vv = e % 4;
-          v[e] = (__bf16)act_fast<ACT>(acc[rt][ct][vv]);
-        }
-        h[ks][ct] = v;
-      }
+      for (int ct = 0; ct < NCT; ++ct) h[ks][ct] = act8<ACT>(acc[2 * ks][ct], acc[2 * ks + 1][ct]);
   };
   activate();
   // layers 2, 3: 128 x 128
 #pragma unroll 1
   for (int layer = 0; layer < 2; ++layer) {
-    const bf16x8_t* F = layer == 0 ? F2 : F3;
+    const mlpx8_t* F = layer == 0 ? F2 : F3;
     const float* bl = sbias + 128 * (layer + 1);
 #pragma unroll
     for (int rt = 0; rt < 8; ++rt) {
@@ -260,10 +299,9 @@ __device__ __forceinline__ void mlp_wave_forward(const bf16x8_t in8[NCT], const 
       for (int ct = 0; ct < NCT; ++ct) acc[rt][ct] = z;
 #pragma unroll
       for (int ks = 0; ks < 4; ++ks) {
-        const bf16x8_t a = F[(rt * 4 + ks) * 64 + lane];
+        const mlpx8_t a = F[(rt * 4 + ks) * 64 + lane];
 #pragma unroll
-        for (int ct = 0; ct < NCT; ++ct)
-          acc[rt][ct] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, h[ks][ct], acc[rt][ct], 0, 0, 0);
+        for (int ct = 0; ct < NCT; ++ct) acc[rt][ct] = mfma16(a, h[ks][ct], acc[rt][ct]);
       }
     }
     activate();
@@ -275,9 +313,9 @@ __device__ __forceinline__ void mlp_wave_forward(const bf16x8_t in8[NCT], const 
   for (int ct = 0; ct < NCT; ++ct) o[ct] = bo;
 #pragma unroll
   for (int ks = 0; ks < 4; ++ks) {
-    const bf16x8_t a = F4[ks * 64 + lane];
+    const mlpx8_t a = F4[ks * 64 + lane];
 #pragma unroll
-    for (int ct = 0; ct < NCT; ++ct) o[ct] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, h[ks][ct], o[ct], 0, 0, 0);
+    for (int ct = 0; ct < NCT; ++ct) o[ct] = mfma16(a, h[ks][ct], o[ct]);
   }
 #pragma unroll
   for (int ct = 0; ct < NCT; ++ct)

@@ -581,7 +581,7 @@ module lpm_direct_f
       integer(c_int32_t) :: ids(*)
     end function
 
-    ! stand-alone MLP (FP32 parameters; BF16 MFMA or FP32 forward)
+    ! stand-alone MLP (FP32 parameters; FP16 MFMA or FP32 forward)
     integer(c_int) function lpm_mlp_new(ctx, mlp, nin, width, nhidden, nout, activation, seed) &
         bind(C, name="lpm_mlp_new")
       import :: c_int, c_ptr, c_int64_t

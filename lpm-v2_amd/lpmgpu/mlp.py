@@ -1,5 +1,5 @@
 """The far-field surrogate network (lpm_mlp_*): FP32 training on the GPU,
-BF16-MFMA inference.  Parameters use the layout W[out][in] then b[out] per layer."""
+FP16-MFMA inference.  Parameters use the layout W[out][in] then b[out] per layer."""
 from __future__ import annotations
 
 import ctypes

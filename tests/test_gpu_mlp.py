@@ -1,5 +1,5 @@
 """The far-field surrogate network on the GPU against the FP64 CPU oracle:
-FP32 forward and gradients (hand-written GEMM backward), BF16-MFMA forward,
+FP32 forward and gradients (hand-written GEMM backward), FP16-MFMA forward,
 and training by the GPU's own Adam."""
 import numpy as np
 import pytest
@@ -32,15 +32,15 @@ def test_forward_fp32_matches_oracle(ctx, oracle, act):
 
 
 @pytest.mark.parametrize("act", ACTS)
-def test_forward_mfma_bf16_matches_oracle(ctx, oracle, act):
+def test_forward_mfma_fp16_matches_oracle(ctx, oracle, act):
     net = Mlp(ctx, NIN, WIDTH, NH, NOUT, seed=4, act=act)
     p = net.params()
     x, _ = _data(1000, seed=1)
     ref = oracle.mlp_forward(p.astype(np.float64), x.astype(np.float64), NIN, WIDTH, NH, NOUT, act)
     got = net.forward(x, mfma=True)
     e = np.sqrt(((got - ref) ** 2).sum() / (ref**2).sum())
-    print("bf16 mfma forward rel rms err", e)
-    assert e < 2e-2  # BF16 inputs/weights/activations, FP32 accumulation
+    print("fp16 mfma forward rel rms err", e)
+    assert e < 3e-3  # FP16 inputs/weights/activations, FP32 accumulation (measured 6e-4 / 7e-4)
 
 
 @pytest.mark.parametrize("act", ACTS)
@@ -69,7 +69,7 @@ def test_training_reduces_loss(ctx):
     l1, _ = net.loss_grad(x[:4096], t[:4096])
     print("loss before / after", l0, l1)
     assert l1 < 0.2 * l0
-    # the packed BF16 copy follows the trained FP32 weights
+    # the packed FP16 copy follows the trained FP32 weights
     y32 = net.forward(x[:2048])
     y16 = net.forward(x[:2048], mfma=True)
     assert np.sqrt(((y16 - y32) ** 2).sum() / (y32**2).sum()) < 3e-2
