@@ -411,7 +411,8 @@ def main():
                     help="oracle threads (default: the CPUs this process may run on, capped by OMP_NUM_THREADS)")
     ap.add_argument("--graph", action="store_true", help="replay steps as hipGraphs (no per-kernel timing)")
     ap.add_argument("--workload", default="n655k",
-                    help="n655k (default, the headline line) or cfg0/cfg1/cfg2/cfg3/pse_ops for the other workloads")
+                    help="n655k (default, the headline line) or cfg0/cfg1/cfg2/cfg3/pse_ops for the other "
+                         "workloads; icos9_splits times the L9 sum at several source-split counts")
     ap.add_argument("--cfg-steps", type=int, default=0, help="override the step count of cfg1/cfg3")
     ap.add_argument("--ff-steps", type=int, default=100, help="learned far-field RK4 steps (0 = skip)")
     ap.add_argument("--ff-samples", type=int, default=10_000_000, help="far-field training samples")
@@ -443,12 +444,13 @@ def main():
 
         ctx = lpmgpu.Context(0)
         fn = {"cfg0": configs.run_cfg0, "cfg1": configs.run_cfg1, "cfg2": configs.run_cfg2,
-              "cfg3": configs.run_cfg3, "pse_ops": configs.run_pse_ops}[args.workload]
+              "cfg3": configs.run_cfg3, "pse_ops": configs.run_pse_ops,
+              "icos9_splits": configs.run_icos9_splits}[args.workload]
         kw = {"steps": args.cfg_steps} if args.cfg_steps and args.workload in ("cfg1", "cfg3") else {}
         res = fn(ctx, log=log, **kw)
         if args.workload == "pse_ops" and args.cpu_seconds > 0:
             pse_ops_cpu_baseline(res, args.cpu_threads)
-        elif args.cpu_seconds > 0:
+        elif args.cpu_seconds > 0 and args.workload != "icos9_splits":
             cfg_cpu_baseline(args.workload, res, args.cpu_seconds, args.cpu_threads)
         res["metric"] = METRIC
         print(json.dumps(res), flush=True)

@@ -292,3 +292,34 @@ def run_pse_ops(ctx: Context, log=print, plane_side=512, sphere_level=7):
                                              precision="fp64"))
     ctx.set_timing(False)
     return res
+
+
+def run_icos9_splits(ctx: Context, log=print, splits=(0, 11, 22, 44, 88, 176)):
+    """One standalone FP32 sphere Biot-Savart sum at icosahedral L8, then the
+    L9 sum (7,864,322 targets, 5,242,880 sources) at the library's split count
+    (0: capped at 2 GiB of FP64 partials) and at forced split counts, each
+    timed on its second launch; results compared with the default count's."""
+    res = {"workload": "icosahedral L8 sum, then L9 sums at several source-split counts, FP32-compensated"}
+    ctx.set_timing(True)
+    for level in (8, 9):
+        w = workloads.cfg4(level=level, cell_level=-1)
+        m, z = w["mesh"], w["relvort"]
+        pairs = m.n * m.n_active - m.n_active
+        sweep = splits if level == 9 else (0,)
+        ref = None
+        out = {"targets": m.n, "sources": m.n_active, "pairs": pairs}
+        for s in sweep:
+            ctx.set_tuning(L.K_SPHERE_BS, "fp32", splits=s)
+            ctx.bve_sphere_velocity(m.x, m.y, m.z, z, m.area, m.active)
+            ctx.reset_kernel_stats()
+            u = np.stack(ctx.bve_sphere_velocity(m.x, m.y, m.z, z, m.area, m.active))
+            k = _kernel(ctx, L.K_SPHERE_BS, FLOPS["sphere"], pairs)
+            ref = u if ref is None else ref
+            k["pairs_per_s"] = round(pairs / (k["avg_ms"] * 1e-3), -6)
+            k["max_rel_diff_vs_default"] = float(np.abs(u - ref).max() / np.abs(ref).max())
+            out[f"splits_{s}"] = k
+            log(f"[icos{level}] splits {s}: {k['avg_ms']} ms")
+        ctx.set_tuning(L.K_SPHERE_BS, "fp32", splits=0)
+        res[f"icos{level}"] = out
+    ctx.set_timing(False)
+    return res
