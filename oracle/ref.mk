@@ -1,9 +1,10 @@
 # Builds lpm-v2's own Fortran modules of the direct-sum path, from the
 # sources where they lie under /root/reference/src (nothing is copied), with
 # amdflang and the image's MPICH (/opt/conda: mpif.h, libmpifort, libmpi),
-# and links the golden-vector driver tests/golden/ref_golden.f90 against
-# them.  Outputs go to oracle/_ref/ only.  Test infrastructure: nothing in
-# the library or the benchmark's timed path uses it.
+# and links the golden-vector driver tests/golden/ref_golden.f90 and the CPU
+# baseline driver oracle/ref_cpu_bench.f90 against them.  Outputs go to
+# oracle/_ref/ only.  Test infrastructure: nothing in the library or the
+# benchmark's timed path uses it.
 #
 #   make -f oracle/ref.mk          (from the repository root)
 #
@@ -20,7 +21,7 @@ MODS     := TypeDefs Utilities STDIntVector OutputWriter Logger SphereGeometry P
             PlaneIncompressibleSolver PSEDirectSum
 OBJS     := $(addprefix $(OUT)/,$(addsuffix .o,$(MODS)))
 
-all: $(OUT)/ref_golden $(OUT)/ref_gpu_timestep
+all: $(OUT)/ref_golden $(OUT)/ref_gpu_timestep $(OUT)/ref_cpu_bench
 
 # The GPU-backed SphereBVESolverModule (lpm-v2_amd/fortran/SphereBVESolver_lpm.f90)
 # linked with the reference's other modules in place of its SphereBVESolver,
@@ -54,6 +55,12 @@ $(OUT)/lpm_ref_modules.stamp: | $(OUT)/mod
 $(OUT)/ref_golden: tests/golden/ref_golden.f90 $(OUT)/lpm_ref_modules.stamp
 	$(AMDFLANG) $(FFLAGS) -c tests/golden/ref_golden.f90 -o $(OUT)/ref_golden.o
 	$(AMDFLANG) -o $@ $(OUT)/ref_golden.o $(OBJS) -L$(MPI)/lib -lmpifort -lmpi -Wl,-rpath,$(MPI)/lib
+
+# The reference's own Biot-Savart loop timed on the host cores (bench.py's
+# CPU baseline, one MPI rank per core).
+$(OUT)/ref_cpu_bench: oracle/ref_cpu_bench.f90 $(OUT)/lpm_ref_modules.stamp
+	$(AMDFLANG) $(FFLAGS) -c oracle/ref_cpu_bench.f90 -o $(OUT)/ref_cpu_bench.o
+	$(AMDFLANG) -o $@ $(OUT)/ref_cpu_bench.o $(OBJS) -L$(MPI)/lib -lmpifort -lmpi -Wl,-rpath,$(MPI)/lib
 
 clean:
 	rm -rf $(OUT)

@@ -17,6 +17,7 @@ import pytest
 from lpmgpu import mesh
 
 HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
 GOLDEN = os.path.join(HERE, "golden")
 
 # The oracle restates the reference's loops in the same FP64 operation order,
@@ -122,3 +123,25 @@ def test_golden_catches_a_planted_mistake(oracle):
     q[j] = -q[j]
     u = oracle.bve_sphere_velocity(g["x"], g["y"], g["z"], q, g["area"], act)
     assert rel(u, (g["u"], g["v"], g["w"])) > 1e3 * TOL
+
+
+@pytest.mark.skipif(not (os.path.exists(os.path.join(ROOT, "oracle", "_ref", "ref_cpu_bench"))
+                         and os.path.exists("/opt/conda/bin/mpiexec")),
+                    reason="the reference's CPU-baseline driver is built only where its sources are "
+                           "(make -f oracle/ref.mk)")
+def test_reference_cpu_baseline_runs_the_reference_loop():
+    """bench.py's CPU baseline runs lpm-v2's own SetVelocityOnMesh on MPI ranks
+    over a block of targets; the block it computes equals the oracle's."""
+    import sys
+
+    sys.path.insert(0, ROOT)
+    import bench
+    from lpmgpu import ic
+
+    m = mesh.icosahedral_mesh(4)
+    z = ic.gaussian_vortex(m.x, m.y, m.z, ic.lat_lon_point(0.3, 0.0), 4 * np.pi, 16.0) + \
+        ic.rh4_vorticity(m.x, m.y, m.z)
+    out = bench.reference_cpu_baseline(m, z, 0.5, 2)
+    assert out is not None and out["kind"] == "reference" and out["cores"] == 2
+    assert out["value"] > 0
+    assert out["max_rel_diff_vs_oracle"] < 1e-12
