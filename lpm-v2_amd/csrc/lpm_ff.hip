@@ -592,11 +592,17 @@ __global__ __launch_bounds__(256) void ff_near_kernel(NearArgs a) {
 #pragma unroll
       for (int r = 0; r < kR; ++r) p[r][0] = p[r][1] = p[r][2] = 0.f;
       if (!checked) {
-        for (int j = jt; j < je; ++j) {
-          const float* sr = a.src + (size_t)j * lpm::SREC;
-          float sv[lpm::SREC];
+        // each source's record is loaded one source ahead of its arithmetic, as
+        // in the plain pair kernel
+        float sv[lpm::SREC];
 #pragma unroll
-          for (int e = 0; e < lpm::SREC; ++e) sv[e] = sr[e];
+        for (int e = 0; e < lpm::SREC; ++e) sv[e] = a.src[(size_t)jt * lpm::SREC + e];
+#pragma unroll 2
+        for (int j = jt; j < je; ++j) {
+          const int jn = j + 1 < je ? j + 1 : j;
+          float sn[lpm::SREC];
+#pragma unroll
+          for (int e = 0; e < lpm::SREC; ++e) sn[e] = a.src[(size_t)jn * lpm::SREC + e];
           lpm::W4<float> w[kR];
           float f[kR];
 #pragma unroll
@@ -605,6 +611,8 @@ __global__ __launch_bounds__(256) void ff_near_kernel(NearArgs a) {
           for (int r = 0; r < kR; ++r) f[r] = P::trans(w[r]);
 #pragma unroll
           for (int r = 0; r < kR; ++r) P::acc(sv, w[r], f[r], p[r]);
+#pragma unroll
+          for (int e = 0; e < lpm::SREC; ++e) sv[e] = sn[e];
         }
       } else {
         for (int j = jt; j < je; ++j) {
