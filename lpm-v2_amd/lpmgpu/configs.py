@@ -116,6 +116,8 @@ def run_cfg2(ctx: Context, log=print):
     inner = r2 < 0.5
     ctx.set_timing(True)
     for order, kind in ((2, L.K_PSE2), (8, L.K_PSE8)):
+        # one untimed call first: the timed launch then runs on warm buffers and clocks
+        ctx.pse_plane_laplacian(w["x"], w["y"], w["f"], w["area"], act, w["eps"], order=order, precision="fp32")
         ctx.reset_kernel_stats()
         t0 = time.perf_counter()
         g32 = ctx.pse_plane_laplacian(w["x"], w["y"], w["f"], w["area"], act, w["eps"], order=order, precision="fp32")
@@ -233,6 +235,7 @@ def run_pse_ops(ctx: Context, log=print, plane_side=512, sphere_level=7):
     ctx.set_timing(True)
 
     def timed(name, kind, flops, pairs, call32, call64):
+        call32()  # untimed: the timed launch runs on warm buffers and clocks
         ctx.reset_kernel_stats()
         t0 = time.perf_counter()
         g32 = call32()
