@@ -133,8 +133,18 @@ def reference_cpu_baseline(m, zeta, target_seconds: float, ranks: int):
         if r.returncode != 0:
             log(f"[cpu reference] exit {r.returncode}: {(r.stdout + r.stderr)[-400:]}")
             return None
-        nr, k, ns, sec, rate = r.stdout.split()[-5:]
-        k, ns, sec, rate = int(k), int(ns), float(sec), float(rate)
+        vals = None
+        for line in reversed(r.stdout.splitlines()):  # the driver's result line: 5 numbers
+            try:
+                nr, k, ns, sec, rate = line.split()
+                vals = int(nr), int(k), int(ns), float(sec), float(rate)
+                break
+            except ValueError:
+                continue
+        if vals is None:
+            log(f"[cpu reference] no result line in: {r.stdout[-400:]}")
+            return None
+        nr, k, ns, sec, rate = vals
         got = np.stack([np.fromfile(os.path.join(tmp, f"out_{c}.f64"), dtype="<f8") for c in "uvw"])
         ref = np.stack([np.asarray(c)[:k] for c in oracle.bve_sphere_velocity(m.x, m.y, m.z, zeta, m.area, m.active,
                                                                                1.0, 0, k)])
