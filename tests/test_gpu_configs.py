@@ -12,6 +12,7 @@ integrals over steps).
   512^2 jittered grid, eps 2h   262,144 x 262,144 (PSE-2 and PSE-8)
   64 problems, icos L3-L7       one persistent launch per evaluation
   icosahedral L8                1,966,082 targets x 1,310,720 sources
+  icosahedral L9                7,864,322 targets x 5,242,880 sources
 """
 import time
 
@@ -222,4 +223,41 @@ def test_cfg4_plain_sum_full_size(ctx, oracle):
     assert e_full < FP32_VEL_TOL
     assert e32 < FP32_VEL_TOL
     assert e64 < FP64_TOL
+    assert udx < 1e-12
+
+
+# ------------------------------------------- icosahedral L9: the partials cap
+def test_icos9_sum_at_the_partials_cap(ctx, oracle):
+    """7,864,322 targets x 5,242,880 sources, the largest sum run here: the
+    split count is capped by 2 GiB of FP64 partials (11 splits of 476,672
+    sources).  FP32 against the FP64 GPU sum on a block of 65,536 targets,
+    both against the oracle on 256 of them, and the capped split count
+    against 88 forced splits over every target."""
+    from lpmgpu import _lib as L
+
+    w = workloads.cfg4(level=9, cell_level=-1)
+    m, zeta = w["mesh"], w["relvort"]
+    assert (m.n, m.n_active) == (7864322, 5242880)
+    print(f"[icos9] T={m.n} S={m.n_active}", flush=True)
+    g32 = ctx.bve_sphere_velocity(m.x, m.y, m.z, zeta, m.area, m.active, precision="fp32")
+    i0, i1 = 3000000, 3065536
+    g64 = ctx.bve_sphere_velocity(m.x, m.y, m.z, zeta, m.area, m.active, i0=i0, i1=i1, precision="fp64")
+    e_blk = rel_err_max(_block(g32, i0, i1), _block(g64, i0, i1))
+    o0, o1 = i0, i0 + 256
+    ref = oracle.bve_sphere_velocity(m.x, m.y, m.z, zeta, m.area, m.active, i0=o0, i1=o1)
+    e32 = rel_err_max(_block(g32, o0, o1), _block(ref, o0, o1))
+    e64 = rel_err_max(_block(g64, o0, o1), _block(ref, o0, o1))
+    ctx.set_tuning(L.K_SPHERE_BS, "fp32", splits=88)
+    try:
+        g88 = ctx.bve_sphere_velocity(m.x, m.y, m.z, zeta, m.area, m.active, precision="fp32")
+    finally:
+        ctx.set_tuning(L.K_SPHERE_BS, "fp32", splits=0)
+    e_split = rel_err_max(g88, g32)
+    udx = _udotx_rel(g32, m)
+    _print("[icos9] velocity", fp32_vs_fp64_block=e_blk, fp32_vs_oracle=e32, fp64_vs_oracle=e64,
+           splits88_vs_capped=e_split, udotx=udx)
+    assert e_blk < FP32_VEL_TOL
+    assert e32 < FP32_VEL_TOL
+    assert e64 < FP64_TOL
+    assert e_split < 1e-11  # same 256-source tiles, FP64 sums of the splits grouped differently
     assert udx < 1e-12
