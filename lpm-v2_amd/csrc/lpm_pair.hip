@@ -222,22 +222,26 @@ static int nout_of(int kind) {
 Tuning default_tuning(int kind, int prec) {
   Tuning t;
   if (prec == LPM_PREC_FP32_KAHAN) {
-    // targets per lane from a sweep of R in {1, 2, 4, 8} x KT in {64 ... 512}
-    // on MI355X (icosahedral L7/L8, 512^2 plane, source chunks of ~2,048):
-    // the Biot-Savart and sphere Green sums run fastest at R = 4, the plane
-    // PSE sums at R = 2, the plane Green sum at R = 8 and the sphere PSE sum
-    // (long series distance, more registers) at R = 1; KT beyond 256 gains
-    // under 1 %
+    // targets per lane from a sweep of R in {1, 2, 4, 8} x KT in {128, 256,
+    // 512} on MI355X (icosahedral L7/L8, 512^2 plane, source chunks of
+    // ~2,048, each source's record loaded one source ahead of its
+    // arithmetic): the sphere Biot-Savart and Green sums run fastest at R = 2
+    // (L7: 36.0 / 35.1 ms against 37.7 / 36.1 at R = 4; the L8 solver's sum
+    // 515 against 548 ms), the plane Biot-Savart and Green sums at R = 4, the
+    // plane PSE sums at R = 2 and the sphere PSE sum (long series distance,
+    // more registers) at R = 1; KT other than 256 changes a launch by under 1 %
     switch (kind) {
-      case LPM_K_PSE8:
-      case LPM_K_PSE2: t.r = 2; break;
-      case LPM_K_PLANE_GREEN: t.r = 8; break;
+      case LPM_K_PLANE_BS:
+      case LPM_K_PLANE_GREEN: t.r = 4; break;
       case LPM_K_PSE_SPHERE8: t.r = 1; break;
-      default: t.r = 4; break;
+      default: t.r = 2; break;
     }
     t.kt = 256;
   } else {
-    t.r = 2;
+    // FP64: R = 2, except the PSE Laplacian sums, 1-2 % faster at R = 4
+    // (512^2 PSE-2 48.3 against 49.0 ms, sphere PSE-8 at L7 199.8 against
+    // 203.1 ms)
+    t.r = (kind == LPM_K_PSE2 || kind == LPM_K_PSE8 || kind == LPM_K_PSE_SPHERE8) ? 4 : 2;
     t.kt = 64;
   }
   return t;
