@@ -487,7 +487,9 @@ int lpm_reset_kernel_stats(lpm_ctx *ctx);
 /* Launch shape of a pair kernel: targets per lane (1, 2, 4 or 8), sources
  * per compensated tile (multiple of 8) and source splits (0 = automatic:
  * chunks of about 2,048 sources, at least enough splits to fill the GPU,
- * at most 2 GiB of FP64 split partials).  Values 0 restore the per-kind
+ * at most 2 GiB of FP64 split partials; a forced count is not capped, and
+ * its partials take splits x outputs x targets x 8 bytes of device memory).
+ * Values 0 restore the per-kind
  * defaults.  The FP64 PSE derivative, interpolation and SWE kinds have no
  * 8-targets-per-lane kernel: their calls then fail with LPM_ERR_ARG.  The
  * shape changes the schedule, not the result beyond FP32 (FP64) rounding. */
