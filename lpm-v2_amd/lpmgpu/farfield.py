@@ -38,10 +38,13 @@ N_PARAMS = 13 * 128 + 128 + 2 * (128 * 128 + 128) + 3 * 128 + 3
 
 class FarFieldBVE:
     """Sphere BVE whose far cells use an expansion of order `order` (2..4) in
-    their moments, with a network correcting the remainder in the band."""
+    their moments, with a network correcting the remainder in the band.  The
+    defaults are the measured icosahedral L8 configuration: cells nearer than
+    0.1 rad summed directly, the network over the far cells within 0.2 rad,
+    training samples taking the nearest of 4 band cells."""
 
-    def __init__(self, ctx, mesh, relvort, rotation_rate=0.0, near_radius=0.1, band_radius=0.45, radius=1.0,
-                 order=4, sample_draws=1):
+    def __init__(self, ctx, mesh, relvort, rotation_rate=0.0, near_radius=0.1, band_radius=0.2, radius=1.0,
+                 order=4, sample_draws=4):
         if mesh.face_cell is None or mesh.cell_level < 0:
             raise ValueError("mesh needs coarse cells (cell_level >= 0)")
         self.ctx = ctx
