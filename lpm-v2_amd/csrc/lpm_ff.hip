@@ -67,6 +67,11 @@ constexpr int kCellRec = 40; // floats per far-field cell record
 constexpr int kMlpRec = 32;  // floats per MLP cell record
 constexpr int kMlpNct = 2;   // 16-target column tiles per network pass
 constexpr int kMlpTile = 16 * kMlpNct;  // targets per network work item
+constexpr int kLocR = 2;  // particles per lane in the binning kernel
+// The near and far kernels split their work (near lists, far cell ranges) in
+// up to kSplitMax parts, enough for about kSplitRounds rounds of resident waves.
+constexpr double kSplitRounds = 6.0;
+constexpr int kSplitMax = 8;
 
 // Symmetric moment tensors are stored by their sorted index tuples in
 // lexicographic order: M2 {00 01 02 11 12 22}, M3 {000 001 002 011 012 022 111
@@ -183,6 +188,68 @@ __device__ __forceinline__ void expansion(const T c[3], T e_c, const T* m, const
   for (int i = 0; i < 3; ++i) G[i] = c[i] * al + be[i];
 }
 
+// The far kernel's FP32 form of expansion<P>, accumulated into p.  Two
+// identities shorten it:
+//   a_k = x . b_k (k >= 1), so alpha = phi (M_0 + x . beta) needs no a_k, and
+//   phi = 2 psi with psi = 1 / (|x-c|^2 + e_c),
+// so with the record's moments scaled by 2^k (M'_k = 2^k M_k / (4 pi R)) and
+// B_k = M'_k[., x^(k-1)]:
+//   beta = psi h,  h = B_1 + psi (B_2 + ... + psi B_P)
+//   alpha = psi (M'_0 + psi (2x) . h)
+//   p += c alpha + psi h.
+// cv = {c (3), e_c, M'_0..M'_P}; x2 = 2x.  84 vector instructions per target
+// and cell at P = 4 against 102 for expansion<4, float> and the sum into p.
+template <int P>
+__device__ __forceinline__ void far_expansion_acc(const float* cv, const float x[3], const float x2[3], float p[3]) {
+  const float dx = x[0] - cv[0], dy = x[1] - cv[1], dz = x[2] - cv[2];
+  const float psi = recip(fmaf(dx, dx, fmaf(dy, dy, fmaf(dz, dz, cv[3]))));
+  const float* m = cv + 4;
+  float B[5][3] = {};
+#pragma unroll
+  for (int k = 0; k < 3; ++k) B[1][k] = m[1 + k];
+  if (P >= 2) {
+#pragma unroll
+    for (int k = 0; k < 3; ++k)
+      B[2][k] = fmaf(m[kMomIdx.b2[k][2]], x[2], fmaf(m[kMomIdx.b2[k][1]], x[1], m[kMomIdx.b2[k][0]] * x[0]));
+  }
+  if (P >= 3) {
+    float q[6];
+#pragma unroll
+    for (int i = 0; i < 6; ++i) q[i] = kPairMul[i] * x[kPair[i][0]] * x[kPair[i][1]];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      float s = m[kMomIdx.b3[k][0]] * q[0];
+#pragma unroll
+      for (int i = 1; i < 6; ++i) s = fmaf(m[kMomIdx.b3[k][i]], q[i], s);
+      B[3][k] = s;
+    }
+  }
+  if (P >= 4) {
+    float q[10];
+#pragma unroll
+    for (int i = 0; i < 10; ++i) q[i] = kTriMul[i] * x[kTri[i][0]] * x[kTri[i][1]] * x[kTri[i][2]];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      float s = m[kMomIdx.b4[k][0]] * q[0];
+#pragma unroll
+      for (int i = 1; i < 10; ++i) s = fmaf(m[kMomIdx.b4[k][i]], q[i], s);
+      B[4][k] = s;
+    }
+  }
+  float h[3];
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    float s = B[P][i];
+#pragma unroll
+    for (int k = P - 1; k >= 1; --k) s = fmaf(psi, s, B[k][i]);
+    h[i] = s;
+  }
+  const float xh = fmaf(x2[0], h[0], fmaf(x2[1], h[1], x2[2] * h[2]));
+  const float alpha = psi * fmaf(psi, xh, m[0]);
+#pragma unroll
+  for (int i = 0; i < 3; ++i) p[i] = fmaf(cv[i], alpha, fmaf(psi, h[i], p[i]));
+}
+
 #pragma clang fp contract(off)
 
 __device__ __forceinline__ void project(double& x, double& y, double& z, double radius) {
@@ -201,23 +268,50 @@ __global__ __launch_bounds__(256) void ff_locate(int n, int ncell, const lpm::Ve
                                                  const double* __restrict__ z, const int* __restrict__ active,
                                                  int* __restrict__ cellof, int* __restrict__ key_t,
                                                  int* __restrict__ key_s, int* __restrict__ iota) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const float px = (float)x[i], py = (float)y[i], pz = (float)z[i];
-  float best = -3.f;
-  int bs = 0;
-  for (int s = 0; s < ncell; ++s) {
-    const lpm::Vec4<float> g = gen[s];
-    const float d = fmaf(g.a, px, fmaf(g.b, py, g.c * pz));
-    if (d > best) {
-      best = d;
-      bs = s;
+  // kLocR particles per lane; generators in groups of 8, so that one wait
+  // covers 8 scalar loads
+  const int i0 = blockIdx.x * blockDim.x * kLocR + threadIdx.x;
+  if (i0 >= n) return;
+  float px[kLocR], py[kLocR], pz[kLocR], best[kLocR];
+  int bs[kLocR];
+#pragma unroll
+  for (int r = 0; r < kLocR; ++r) {
+    const int i = min(i0 + r * (int)blockDim.x, n - 1);
+    px[r] = (float)x[i];
+    py[r] = (float)y[i];
+    pz[r] = (float)z[i];
+    best[r] = -3.f;
+    bs[r] = 0;
+  }
+  auto visit = [&](const lpm::Vec4<float>& g, int s) {
+#pragma unroll
+    for (int r = 0; r < kLocR; ++r) {
+      const float d = fmaf(g.a, px[r], fmaf(g.b, py[r], g.c * pz[r]));
+      if (d > best[r]) {  // strict: the lowest index wins a tie
+        best[r] = d;
+        bs[r] = s;
+      }
+    }
+  };
+  int s = 0;
+  for (; s + 8 <= ncell; s += 8) {
+    lpm::Vec4<float> g[8];
+#pragma unroll
+    for (int e = 0; e < 8; ++e) g[e] = gen[s + e];
+#pragma unroll
+    for (int e = 0; e < 8; ++e) visit(g[e], s + e);
+  }
+  for (; s < ncell; ++s) visit(gen[s], s);
+#pragma unroll
+  for (int r = 0; r < kLocR; ++r) {
+    const int i = i0 + r * (int)blockDim.x;
+    if (i < n) {
+      cellof[i] = bs[r];
+      key_t[i] = bs[r];
+      key_s[i] = active[i] ? bs[r] : ncell;
+      iota[i] = i;
     }
   }
-  cellof[i] = bs;
-  key_t[i] = bs;
-  key_s[i] = active[i] ? bs : ncell;
-  iota[i] = i;
 }
 
 __device__ __forceinline__ int lower_bound_dev(const int* __restrict__ a, int n, int key) {
@@ -296,7 +390,7 @@ struct GeomArgs {
   const int *src_perm, *src_start, *tgt_perm, *tgt_start;
   const double *x, *y, *z, *q, *area;
   double radius;
-  float* far_rec;      // [ncell][40]: c(3), e_c, M0..M4 / (4 pi R) (35), rho
+  float* far_rec;      // [ncell][40]: c(3), e_c, M0..M4 (35) / (4 pi R) with M0 scaled by 2 and Mk by 2^k, rho
   float* mlp_rec;      // [ncell][32]: c(3), 1/l, Q(9), s_out, pad(2), features 3..12, pad(6)
   const double* adir;  // [ncell][3] unit anchor direction of each cell's frame
   double* geo;         // [ncell][8]: src dir (3), src rho, tgt dir (3), tgt rho
@@ -412,7 +506,8 @@ __global__ __launch_bounds__(256) void ff_cell_geometry(GeomArgs a) {
     fr[1] = (float)cy;
     fr[2] = (float)cz;
     fr[3] = (float)(R * R - (cx * cx + cy * cy + cz * cz));
-    for (int k = 0; k < kNMom; ++k) fr[4 + k] = (float)(m[k] * inv);
+    // M_0 and M_1 scaled by 2, M_2 by 4, M_3 by 8, M_4 by 16 (far_expansion_acc)
+    for (int k = 0; k < kNMom; ++k) fr[4 + k] = (float)(m[k] * inv * (k < 4 ? 2.0 : k < 10 ? 4.0 : k < 20 ? 8.0 : 16.0));
     fr[4 + kNMom] = (float)rho;
     const double l = fmax(2.0 * R * sin(0.5 * rho), 1e-12);
     const double W = wabs + 1e-300;
@@ -486,7 +581,8 @@ struct ClassArgs {
   int ncell;
   const double* geo;
   double r_near, r_band;
-  unsigned char* cls;  // [ncell][ncell]
+  unsigned long long* near_mask;  // [ncell][mask_words]: bit s of row t set when s is near t
+  int mask_words;                 // ceil(ncell / 64)
   int* near_list;      // [ncell][ncell]
   int* near_count;
   int* band_list;      // [ncell][ncell]
@@ -511,7 +607,6 @@ __global__ __launch_bounds__(256) void ff_classify(ClassArgs a) {
       const double ang = acos(fmin(1.0, fmax(-1.0, cosang)));
       const double pad = gt[7] + gs[3];
       cls = ang < a.r_near + pad ? 2 : (ang < a.r_band + pad ? 1 : 0);
-      a.cls[(size_t)t * a.ncell + s] = (unsigned char)cls;
     }
     // ordered compaction: one wave at a time keeps lists sorted by s
     for (int w = 0; w < 4; ++w) {
@@ -524,6 +619,7 @@ __global__ __launch_bounds__(256) void ff_classify(ClassArgs a) {
         if (lane == 0) {
           nn += __popcll(mn);
           nb += __popcll(mb);
+          if (base + 64 * w < a.ncell) a.near_mask[(size_t)t * a.mask_words + (base >> 6) + w] = mn;
         }
       }
       __syncthreads();
@@ -582,12 +678,17 @@ __global__ __launch_bounds__(256) void ff_near_kernel(NearArgs a) {
   const int split = blockIdx.y;
   const int qb = (int)((long long)nn * split / a.nsplit), qe = (int)((long long)nn * (split + 1) / a.nsplit);
   double* out = a.out + (size_t)split * 3 * a.nt;
+  // particle ids of the wave's targets: ascending inside a cell (stable binning)
+  const int id_lo = a.tgt_perm[k0], id_hi = a.tgt_perm[min(k0 + 64 * kR, k_end) - 1];
   for (int q = qb; q < qe; ++q) {
     const int s = __builtin_amdgcn_readfirstlane(a.near_list[(size_t)t * a.ncell + q]);
     const int j0 = a.src_start[s], j1 = a.src_start[s + 1];
-    const bool checked = (s == t);  // a target's own source lives in its own cell
     for (int jt = j0; jt < j1; jt += kKT) {
       const int je = min(j1, jt + kKT);
+      // a target's own source lives in its own cell, in the source tile whose
+      // (ascending) particle ids span the target's
+      const bool checked = s == t && __float_as_int(a.src[(size_t)jt * lpm::SREC + P::IDX]) <= id_hi &&
+                           __float_as_int(a.src[(size_t)(je - 1) * lpm::SREC + P::IDX]) >= id_lo;
       float p[kR][3];
 #pragma unroll
       for (int r = 0; r < kR; ++r) p[r][0] = p[r][1] = p[r][2] = 0.f;
@@ -650,21 +751,23 @@ __global__ __launch_bounds__(256) void ff_near_kernel(NearArgs a) {
 }
 
 // ------------------------------------------------------- far (expansion)
+// Every cell that is not near a target's cell, by its expansion.  The cells
+// are split in gridDim.y contiguous ranges, one FP64 partial per range, so
+// that the launch has several rounds of waves and its last round is short.
 struct FarArgs {
   const lpm::Vec4<float>* tgt;
   const int* tgt_start;
   const float* cell;          // far records
-  const unsigned char* cls;
+  const unsigned long long* near_mask;
+  int mask_words;
   const int2* items;
   const int* n_items;  // device count
   int ncell, nt;
-  const double* near_part;    // [nsplit][3][nt] near-field partials
-  int nsplit;
-  double* out;                // [3][nt] = near partials (fixed order) + far
+  double* part;               // [gridDim.y][3][nt] this launch's partials
 };
 
 template <int P>
-__global__ __launch_bounds__(256) void ff_far_kernel(FarArgs a) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7))) void ff_far_kernel(FarArgs a) {
   // the wave's item index is wave-uniform: readfirstlane puts it (and every
   // index and record address derived from it) in scalar registers
   const int item = __builtin_amdgcn_readfirstlane(blockIdx.x * 4 + (threadIdx.x >> 6));
@@ -674,7 +777,7 @@ __global__ __launch_bounds__(256) void ff_far_kernel(FarArgs a) {
   const int t = it.x;
   const int k0 = a.tgt_start[t] + it.y * 64 * kR;
   const int k_end = a.tgt_start[t + 1];
-  float X[kR][3];
+  float X[kR][3], X2[kR][3];
 #pragma unroll
   for (int r = 0; r < kR; ++r) {
     const int k = k0 + 64 * r + lane;
@@ -682,49 +785,60 @@ __global__ __launch_bounds__(256) void ff_far_kernel(FarArgs a) {
     X[r][0] = v.a;
     X[r][1] = v.b;
     X[r][2] = v.c;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) X2[r][c] = 2.f * X[r][c];
   }
   float S[kR][3], C[kR][3];
 #pragma unroll
   for (int r = 0; r < kR; ++r)
 #pragma unroll
     for (int c = 0; c < 3; ++c) S[r][c] = C[r][c] = 0.f;
-  const unsigned char* row = a.cls + (size_t)t * a.ncell;
-  for (int s0 = 0; s0 < a.ncell; s0 += 64) {
-    const int s1 = min(a.ncell, s0 + 64);
+  const unsigned long long* near = a.near_mask + (size_t)t * a.mask_words;
+  const int c_lo = (int)((long long)a.ncell * blockIdx.y / gridDim.y);
+  const int c_hi = (int)((long long)a.ncell * (blockIdx.y + 1) / gridDim.y);
+  // one compensated block per 64-cell word of the mask; the far cells of a
+  // word are its clear bits inside [c_lo, c_hi), visited lowest first
+  for (int wd = c_lo >> 6; wd < (c_hi + 63) >> 6; ++wd) {
+    const int b_lo = max(c_lo - 64 * wd, 0), b_hi = min(c_hi - 64 * wd, 64);
+    const unsigned long long span = (b_hi == 64 ? ~0ull : (1ull << b_hi) - 1) & ~((1ull << b_lo) - 1);
+    unsigned long long far = ~near[wd] & span;
     float p[kR][3];
 #pragma unroll
     for (int r = 0; r < kR; ++r) p[r][0] = p[r][1] = p[r][2] = 0.f;
-    for (int s = s0; s < s1; ++s) {
-      if (row[s] == 2) continue;  // near: summed directly (wave-uniform branch)
+    while (far) {
+      const int s = 64 * wd + __builtin_ctzll(far);
+      far &= far - 1;
       const float* cr = a.cell + (size_t)s * kCellRec;
-      constexpr int NV = 4 + kMomOff[P] + (P + 1) * (P + 2) / 2;  // c, e_c, M0..MP
+      constexpr int NV = 4 + kMomOff[P] + (P + 1) * (P + 2) / 2;  // c, e_c, M'0..M'P
       float cv[NV];
 #pragma unroll
       for (int e = 0; e < NV; ++e) cv[e] = cr[e];
 #pragma unroll
-      for (int r = 0; r < kR; ++r) {
-        float G[3];
-        expansion<P, float>(cv, cv[3], cv + 4, X[r], G);
-#pragma unroll
-        for (int c = 0; c < 3; ++c) p[r][c] += G[c];
-      }
+      for (int r = 0; r < kR; ++r) far_expansion_acc<P>(cv, X[r], X2[r], p[r]);
     }
 #pragma unroll
     for (int r = 0; r < kR; ++r)
 #pragma unroll
       for (int c = 0; c < 3; ++c) lpm::two_sum_acc(S[r][c], C[r][c], p[r][c]);
   }
+  double* out = a.part + (size_t)blockIdx.y * 3 * a.nt;
 #pragma unroll
   for (int r = 0; r < kR; ++r) {
     const int k = k0 + 64 * r + lane;
     if (k < k_end)
 #pragma unroll
-      for (int c = 0; c < 3; ++c) {
-        double v = 0.0;
-        for (int sp = 0; sp < a.nsplit; ++sp) v += a.near_part[((size_t)sp * 3 + c) * a.nt + k];
-        a.out[(size_t)c * a.nt + k] = v + ((double)S[r][c] + (double)C[r][c]);
-      }
+      for (int c = 0; c < 3; ++c) out[(size_t)c * a.nt + k] = (double)S[r][c] + (double)C[r][c];
   }
+}
+
+// V = the near partials, then the far partials, summed in this fixed order.
+__global__ __launch_bounds__(256) void ff_sum_parts(int nt, int nparts, const double* __restrict__ part,
+                                                    double* __restrict__ V) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= 3 * nt) return;
+  double v = part[k];
+  for (int sp = 1; sp < nparts; ++sp) v += part[(size_t)sp * 3 * nt + k];
+  V[k] = v;
 }
 
 // --------------------------------------------------- MLP correction (MFMA)
@@ -1063,11 +1177,12 @@ struct lpm_bve_ff {
   int end_bit = 0;
   size_t sort_bytes = 0;
   DevBuf trec, srec, far_rec, mlp_rec, geo, cen64, mom64;
-  DevBuf cls, near_list, near_count, band_list, band_count;
+  DevBuf near_mask, near_list, near_count, band_list, band_count;
   DevBuf items, mlp_items;
   DevBuf V;
-  DevBuf Vnear;        // near-field partials [near_split][3][n]
+  DevBuf Vpart;        // FP64 partials [near_split + far_split][3][n]: near, then far
   int near_split = 1;  // near lists split in this many parts (fills the chip)
+  int far_split = 1;   // far cells split in this many ranges
   DevBuf list_plain;  // ascending active list for the plain path
   DevBuf tgt_plain, src_plain, partial_plain;
   DevBuf ptrs, err_part;
@@ -1095,7 +1210,7 @@ int ff_upload(lpm_ctx* ctx, DevBuf& b, const void* host, size_t bytes) {
 int ff_bin(lpm_bve_ff* f, const double* x, const double* y, const double* z) {
   lpm_ctx* ctx = f->ctx;
   const int n = f->n, nc = f->ncell;
-  hipLaunchKernelGGL(ff_locate, dim3((n + 255) / 256), dim3(256), 0, ctx->stream, n, nc,
+  hipLaunchKernelGGL(ff_locate, dim3((n + 256 * kLocR - 1) / (256 * kLocR)), dim3(256), 0, ctx->stream, n, nc,
                      f->gen.as<lpm::Vec4<float>>(), x, y, z, f->active.as<int>(), f->cellof.as<int>(),
                      f->key_t.as<int>(), f->key_s.as<int>(), f->iota.as<int>());
   size_t bytes = f->sort_bytes;
@@ -1154,12 +1269,12 @@ int ff_eval(lpm_bve_ff* f, int mode, const double* x, const double* y, const dou
               f->adir.as<double>(), f->geo.as<double>(), f->cen64.as<double>(), f->mom64.as<double>()};
   hipLaunchKernelGGL(ff_cell_geometry, dim3(nc), b256, 0, ctx->stream, ga);
   ClassArgs ca{nc, f->geo.as<double>(), f->r_near, f->r_band,
-               f->cls.as<unsigned char>(), f->near_list.as<int>(),
+               f->near_mask.as<unsigned long long>(), (nc + 63) / 64, f->near_list.as<int>(),
                f->near_count.as<int>(), f->band_list.as<int>(), f->band_count.as<int>()};
   hipLaunchKernelGGL(ff_classify, dim3(nc), b256, 0, ctx->stream, ca);
   NearArgs na{f->trec.as<lpm::Vec4<float>>(), f->tgt_perm.as<int>(), f->tgt_start.as<int>(), f->srec.as<float>(),
               f->src_start.as<int>(), f->near_list.as<int>(), f->near_count.as<int>(), f->items.as<int2>(),
-              f->item_count.as<int>(), nc, f->Vnear.as<double>(), f->n, f->near_split};
+              f->item_count.as<int>(), nc, f->Vpart.as<double>(), f->n, f->near_split};
   hipEvent_t e0 = nullptr, e1 = nullptr;
   if (ctx->timing) {
     e0 = lpm::take_event(ctx);
@@ -1167,13 +1282,15 @@ int ff_eval(lpm_bve_ff* f, int mode, const double* x, const double* y, const dou
     if (e0 && e1) LPM_HIP(ctx, hipEventRecord(e0, ctx->stream));
   }
   hipLaunchKernelGGL(ff_near_kernel, dim3((f->n_items + 3) / 4, f->near_split), b256, 0, ctx->stream, na);
-  FarArgs fa{f->trec.as<lpm::Vec4<float>>(), f->tgt_start.as<int>(), f->far_rec.as<float>(), f->cls.as<unsigned char>(),
-             f->items.as<int2>(), f->item_count.as<int>(), nc, f->n, f->Vnear.as<double>(), f->near_split,
-             f->V.as<double>()};
-  const dim3 gfar((f->n_items + 3) / 4);
+  FarArgs fa{f->trec.as<lpm::Vec4<float>>(), f->tgt_start.as<int>(), f->far_rec.as<float>(),
+             f->near_mask.as<unsigned long long>(), (nc + 63) / 64, f->items.as<int2>(), f->item_count.as<int>(), nc,
+             f->n, f->Vpart.as<double>() + (size_t)f->near_split * 3 * f->n};
+  const dim3 gfar((f->n_items + 3) / 4, f->far_split);
   if (f->order == 2) hipLaunchKernelGGL(ff_far_kernel<2>, gfar, b256, 0, ctx->stream, fa);
   else if (f->order == 3) hipLaunchKernelGGL(ff_far_kernel<3>, gfar, b256, 0, ctx->stream, fa);
   else hipLaunchKernelGGL(ff_far_kernel<4>, gfar, b256, 0, ctx->stream, fa);
+  hipLaunchKernelGGL(ff_sum_parts, dim3((3 * f->n + 255) / 256), b256, 0, ctx->stream, f->n,
+                     f->near_split + f->far_split, f->Vpart.as<double>(), f->V.as<double>());
   if (mode == 2 && f->trained) {
     MlpCorrArgs ma{f->trec.as<lpm::Vec4<float>>(), f->tgt_start.as<int>(), f->mlp_rec.as<float>(),
                    f->band_list.as<int>(), f->band_count.as<int>(), f->mlp_items.as<int2>(), f->item_count.as<int>() + 1, nc, f->n,
@@ -1256,16 +1373,20 @@ int ff_load(lpm_bve_ff* f, int n, const double* x, const double* y, const double
   // work-item upper bounds: per cell ceil(count / tile) <= count / tile + 1
   f->n_items = ncell + n / (64 * kR) + 1;
   f->n_mlp_items = ncell + n / kMlpTile + 1;
-  // near-list splits: about three rounds of the near kernel's resident waves
+  // near-list and far-cell splits: about kSplitRounds rounds of each kernel's
+  // resident waves
   {
-    int nb = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, reinterpret_cast<const void*>(ff_near_kernel), 256, 0) !=
-            hipSuccess ||
-        nb <= 0)
-      nb = 1;
     const double waves = (double)n / (64 * kR) + 0.5 * ncell;
-    const double slots = (double)nb * 4 * ctx->num_cus;
-    f->near_split = std::min(4, std::max(1, (int)std::ceil(3.0 * slots / std::max(waves, 1.0))));
+    auto split_for = [&](const void* kernel) {
+      int nb = 0;
+      if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, kernel, 256, 0) != hipSuccess || nb <= 0) nb = 1;
+      const double slots = (double)nb * 4 * ctx->num_cus;
+      return std::min(kSplitMax, std::max(1, (int)std::ceil(kSplitRounds * slots / std::max(waves, 1.0))));
+    };
+    f->near_split = split_for(reinterpret_cast<const void*>(ff_near_kernel));
+    f->far_split = split_for(f->order == 2   ? reinterpret_cast<const void*>(ff_far_kernel<2>)
+                             : f->order == 3 ? reinterpret_cast<const void*>(ff_far_kernel<3>)
+                                             : reinterpret_cast<const void*>(ff_far_kernel<4>));
   }
   const int end_bit = 32 - __builtin_clz((unsigned)ncell);  // keys 0..ncell
   size_t sort_bytes = 0;
@@ -1308,12 +1429,12 @@ int ff_load(lpm_bve_ff* f, int n, const double* x, const double* y, const double
       chk(ff_upload(ctx, f->geo, nullptr, (size_t)ncell * 8 * 8)) &&
       chk(ff_upload(ctx, f->cen64, nullptr, (size_t)ncell * 3 * 8)) &&
       chk(ff_upload(ctx, f->mom64, nullptr, (size_t)ncell * kNMom * 8)) &&
-      chk(ff_upload(ctx, f->cls, nullptr, (size_t)ncell * ncell)) &&
+      chk(ff_upload(ctx, f->near_mask, nullptr, (size_t)ncell * ((ncell + 63) / 64) * 8)) &&
       chk(ff_upload(ctx, f->near_list, nullptr, (size_t)ncell * ncell * 4)) &&
       chk(ff_upload(ctx, f->near_count, nullptr, (size_t)ncell * 4)) &&
       chk(ff_upload(ctx, f->band_list, nullptr, (size_t)ncell * ncell * 4)) &&
       chk(ff_upload(ctx, f->band_count, nullptr, (size_t)ncell * 4)) && chk(ff_upload(ctx, f->V, nullptr, 3 * nb)) &&
-      chk(ff_upload(ctx, f->Vnear, nullptr, (size_t)f->near_split * 3 * nb));
+      chk(ff_upload(ctx, f->Vpart, nullptr, (size_t)(f->near_split + f->far_split) * 3 * nb));
   for (int s = 0; s < 4 && rc == LPM_OK; ++s)
     for (int c = 0; c < 4 && rc == LPM_OK; ++c) chk(ff_upload(ctx, f->k[s][c], nullptr, nb));
   if (rc == LPM_OK) {
@@ -1696,11 +1817,11 @@ int lpm_bve_ff_delete(lpm_bve_ff* f) {
   (void)hipStreamSynchronize(f->ctx->stream);
   DevBuf* all[] = {&f->x, &f->y, &f->z, &f->q, &f->u, &f->v, &f->w, &f->area, &f->xi, &f->yi, &f->zi, &f->qi,
                    &f->pu, &f->pv, &f->pw, &f->src_perm, &f->src_start, &f->tgt_perm, &f->tgt_start, &f->tgt_cell,
-                   &f->trec, &f->srec, &f->far_rec, &f->mlp_rec, &f->geo, &f->cen64, &f->mom64, &f->cls,
+                   &f->trec, &f->srec, &f->far_rec, &f->mlp_rec, &f->geo, &f->cen64, &f->mom64, &f->near_mask,
                    &f->near_list, &f->near_count, &f->band_list, &f->band_count, &f->items, &f->mlp_items, &f->V,
                    &f->list_plain, &f->tgt_plain, &f->src_plain, &f->partial_plain, &f->ptrs, &f->err_part, &f->sx,
                    &f->sy, &f->sid, &f->src_key, &f->key_t, &f->key_s, &f->iota, &f->cellof, &f->active,
-                   &f->sort_tmp, &f->item_count, &f->gen, &f->adir, &f->Vnear};
+                   &f->sort_tmp, &f->item_count, &f->gen, &f->adir, &f->Vpart};
   for (DevBuf* b : all) b->release();
   for (auto& row : f->k)
     for (auto& b : row) b.release();
