@@ -473,6 +473,21 @@ int lpm_bve_ff_set_state(lpm_bve_ff *ff, const double *x, const double *y, const
 int lpm_bve_ff_particle_cells(lpm_bve_ff *ff, int32_t *cell);
 /* Near and band cell counts per target cell (length ncell each). */
 int lpm_bve_ff_cells(lpm_bve_ff *ff, int *near_cells, int *band_cells);
+/* Coarse level of the expansion part (parent_radius > 0; 0 turns it off, the
+ * default).  Cells 4p .. 4p+3 must be the four children of one coarser cell p,
+ * as mesh refinement numbers them (ncell divisible by 4).  For target cell t,
+ * four sibling cells that are all far (outside the band) are replaced by one
+ * order-P expansion of their union about its own centroid when the angle
+ * between t's and the union's directions is at least parent_radius + rho_t +
+ * rho_union; near and band cells, and every group that holds one, keep their
+ * own treatment.  Takes effect at the next evaluation.  Measured at
+ * icosahedral L8 with 1,280 cells, near 0.1, band 0.2: parent_radius 0.6
+ * evaluates 462 instead of 1,258 expansions per target. */
+int lpm_bve_ff_set_parents(lpm_bve_ff *ff, double parent_radius);
+/* Expansion records evaluated per target cell by the last evaluation (length
+ * ncell): far and band cells, with the coarse level their parents in their
+ * place where it applies. */
+int lpm_bve_ff_far_counts(lpm_bve_ff *ff, int *far_records);
 int lpm_bve_ff_delete(lpm_bve_ff *ff);
 
 /* ----------------------------------------------------- timing / tuning -- */

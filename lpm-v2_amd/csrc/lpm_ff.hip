@@ -422,19 +422,25 @@ __device__ double block_max(double v, double* sh) {
   return r;
 }
 
-__global__ __launch_bounds__(256) void ff_cell_geometry(GeomArgs a) {
-  __shared__ double sh[256];
-  const int c = blockIdx.x;
-  const int s0 = a.src_start[c], s1 = a.src_start[c + 1];
-  const int t0 = a.tgt_start[c], t1 = a.tgt_start[c + 1];
-  // area-weighted source centroid
+// Area-weighted centroid c of the sources [s0, s1) (cell order), their
+// moments M0..M4 of w = zeta A about c, sum |w| and angular radius about c's
+// direction.  Every thread of the block gets the block's values.
+struct SourceMoments {
+  double cx, cy, cz, ux, uy, uz, rho, wabs;
+  double m[kNMom];
+};
+
+__device__ void source_moments(const int* __restrict__ src_perm, const double* __restrict__ x,
+                               const double* __restrict__ y, const double* __restrict__ z,
+                               const double* __restrict__ q, const double* __restrict__ area, int s0, int s1,
+                               double* sh, SourceMoments& r) {
   double ax = 0, ay = 0, az = 0, aa = 0;
   for (int j = s0 + threadIdx.x; j < s1; j += 256) {
-    const int p = a.src_perm[j];
-    const double w = a.area[p];
-    ax += w * a.x[p];
-    ay += w * a.y[p];
-    az += w * a.z[p];
+    const int p = src_perm[j];
+    const double w = area[p];
+    ax += w * x[p];
+    ay += w * y[p];
+    az += w * z[p];
     aa += w;
   }
   ax = block_sum(ax, sh);
@@ -444,14 +450,13 @@ __global__ __launch_bounds__(256) void ff_cell_geometry(GeomArgs a) {
   const double cx = aa > 0 ? ax / aa : 0.0, cy = aa > 0 ? ay / aa : 0.0, cz = aa > 0 ? az / aa : 1.0;
   const double cn = sqrt(cx * cx + cy * cy + cz * cz);
   const double ux = cx / cn, uy = cy / cn, uz = cz / cn;
-  // moments M0..M4 about c, radius
-  double m[kNMom];
+  double* m = r.m;
   for (int k = 0; k < kNMom; ++k) m[k] = 0.0;
   double wabs = 0.0, rho = 0.0;
   for (int j = s0 + threadIdx.x; j < s1; j += 256) {
-    const int p = a.src_perm[j];
-    const double w = a.q[p] * a.area[p];
-    const double d[3] = {a.x[p] - cx, a.y[p] - cy, a.z[p] - cz};
+    const int p = src_perm[j];
+    const double w = q[p] * area[p];
+    const double d[3] = {x[p] - cx, y[p] - cy, z[p] - cz};
     m[0] += w;
 #pragma unroll
     for (int i = 0; i < 3; ++i) m[1 + i] += w * d[i];
@@ -463,14 +468,45 @@ __global__ __launch_bounds__(256) void ff_cell_geometry(GeomArgs a) {
     for (int i = 0; i < 15; ++i)
       m[kMomOff[4] + i] += w * d[kQuad[i][0]] * d[kQuad[i][1]] * d[kQuad[i][2]] * d[kQuad[i][3]];
     wabs += fabs(w);
-    const double r = sqrt(a.x[p] * a.x[p] + a.y[p] * a.y[p] + a.z[p] * a.z[p]);
-    const double cosang = (a.x[p] * ux + a.y[p] * uy + a.z[p] * uz) / r;
+    const double rr = sqrt(x[p] * x[p] + y[p] * y[p] + z[p] * z[p]);
+    const double cosang = (x[p] * ux + y[p] * uy + z[p] * uz) / rr;
     rho = fmax(rho, acos(fmin(1.0, fmax(-1.0, cosang))));
   }
 #pragma unroll
   for (int k = 0; k < kNMom; ++k) m[k] = block_sum(m[k], sh);
-  wabs = block_sum(wabs, sh);
-  rho = block_max(rho, sh);
+  r.wabs = block_sum(wabs, sh);
+  r.rho = block_max(rho, sh);
+  r.cx = cx;
+  r.cy = cy;
+  r.cz = cz;
+  r.ux = ux;
+  r.uy = uy;
+  r.uz = uz;
+}
+
+// The FP32 far record of far_expansion_acc: c, e_c = R^2 - |c|^2, the moments
+// / (4 pi R) with M_0 and M_1 scaled by 2, M_2 by 4, M_3 by 8, M_4 by 16, rho.
+__device__ void write_far_record(float* fr, const SourceMoments& sm, double R) {
+  const double inv = 1.0 / (4.0 * kPi * R);
+  fr[0] = (float)sm.cx;
+  fr[1] = (float)sm.cy;
+  fr[2] = (float)sm.cz;
+  fr[3] = (float)(R * R - (sm.cx * sm.cx + sm.cy * sm.cy + sm.cz * sm.cz));
+  for (int k = 0; k < kNMom; ++k)
+    fr[4 + k] = (float)(sm.m[k] * inv * (k < 4 ? 2.0 : k < 10 ? 4.0 : k < 20 ? 8.0 : 16.0));
+  fr[4 + kNMom] = (float)sm.rho;
+}
+
+__global__ __launch_bounds__(256) void ff_cell_geometry(GeomArgs a) {
+  __shared__ double sh[256];
+  const int c = blockIdx.x;
+  const int s0 = a.src_start[c], s1 = a.src_start[c + 1];
+  const int t0 = a.tgt_start[c], t1 = a.tgt_start[c + 1];
+  SourceMoments sm;
+  source_moments(a.src_perm, a.x, a.y, a.z, a.q, a.area, s0, s1, sh, sm);
+  const double cx = sm.cx, cy = sm.cy, cz = sm.cz, ux = sm.ux, uy = sm.uy, uz = sm.uz;
+  const double* m = sm.m;
+  const double wabs = sm.wabs, rho = sm.rho;
   // target cell direction and radius
   double tx = 0, ty = 0, tz = 0;
   for (int k = t0 + threadIdx.x; k < t1; k += 256) {
@@ -501,14 +537,7 @@ __global__ __launch_bounds__(256) void ff_cell_geometry(GeomArgs a) {
   if (threadIdx.x == 0) {
     const double R = a.radius;
     const double inv = 1.0 / (4.0 * kPi * R);
-    float* fr = a.far_rec + (size_t)c * kCellRec;
-    fr[0] = (float)cx;
-    fr[1] = (float)cy;
-    fr[2] = (float)cz;
-    fr[3] = (float)(R * R - (cx * cx + cy * cy + cz * cz));
-    // M_0 and M_1 scaled by 2, M_2 by 4, M_3 by 8, M_4 by 16 (far_expansion_acc)
-    for (int k = 0; k < kNMom; ++k) fr[4 + k] = (float)(m[k] * inv * (k < 4 ? 2.0 : k < 10 ? 4.0 : k < 20 ? 8.0 : 16.0));
-    fr[4 + kNMom] = (float)rho;
+    write_far_record(a.far_rec + (size_t)c * kCellRec, sm, R);
     const double l = fmax(2.0 * R * sin(0.5 * rho), 1e-12);
     const double W = wabs + 1e-300;
     // cell frame: e3 = c/|c|, e1 = tangential part of (R anchor - c), e2 = e3 x e1
@@ -575,14 +604,47 @@ __global__ __launch_bounds__(256) void ff_cell_geometry(GeomArgs a) {
   }
 }
 
+// Coarse level: parent p is the four cells 4p .. 4p+3, whose sources are
+// contiguous in cell order.  Its far record (after the cells' records) and its
+// geometry {direction of c (3), rho}.
+struct ParentArgs {
+  const int *src_perm, *src_start;
+  const double *x, *y, *z, *q, *area;
+  double radius;
+  float* far_rec;  // [npar][kCellRec]
+  double* pgeo;    // [npar][4]
+};
+
+__global__ __launch_bounds__(256) void ff_parent_geometry(ParentArgs a) {
+  __shared__ double sh[256];
+  const int p = blockIdx.x;
+  SourceMoments sm;
+  source_moments(a.src_perm, a.x, a.y, a.z, a.q, a.area, a.src_start[4 * p], a.src_start[4 * p + 4], sh, sm);
+  if (threadIdx.x == 0) {
+    write_far_record(a.far_rec + (size_t)p * kCellRec, sm, a.radius);
+    double* g = a.pgeo + (size_t)p * 4;
+    g[0] = sm.ux;
+    g[1] = sm.uy;
+    g[2] = sm.uz;
+    g[3] = sm.rho;
+  }
+}
+
+// Class of source cell s for target cell t (0 far, 1 band, 2 near) from
+// their FP64 geometry rows (geo: src dir (3), src rho, tgt dir (3), tgt rho).
+__device__ __forceinline__ int cell_class(const double* gt, const double* gs, double r_near, double r_band) {
+  const double cosang = gt[4] * gs[0] + gt[5] * gs[1] + gt[6] * gs[2];
+  const double ang = acos(fmin(1.0, fmax(-1.0, cosang)));
+  const double pad = gt[7] + gs[3];
+  return ang < r_near + pad ? 2 : (ang < r_band + pad ? 1 : 0);
+}
+
 // Per target cell t: class of every source cell (0 far, 1 band, 2 near) and
 // compact lists of near and band cells.
 struct ClassArgs {
   int ncell;
   const double* geo;
   double r_near, r_band;
-  unsigned long long* near_mask;  // [ncell][mask_words]: bit s of row t set when s is near t
-  int mask_words;                 // ceil(ncell / 64)
   int* near_list;      // [ncell][ncell]
   int* near_count;
   int* band_list;      // [ncell][ncell]
@@ -601,13 +663,7 @@ __global__ __launch_bounds__(256) void ff_classify(ClassArgs a) {
   for (int base = 0; base < a.ncell; base += 256) {
     const int s = base + threadIdx.x;
     int cls = -1;
-    if (s < a.ncell) {
-      const double* gs = a.geo + (size_t)s * 8;
-      const double cosang = gt[4] * gs[0] + gt[5] * gs[1] + gt[6] * gs[2];
-      const double ang = acos(fmin(1.0, fmax(-1.0, cosang)));
-      const double pad = gt[7] + gs[3];
-      cls = ang < a.r_near + pad ? 2 : (ang < a.r_band + pad ? 1 : 0);
-    }
+    if (s < a.ncell) cls = cell_class(gt, a.geo + (size_t)s * 8, a.r_near, a.r_band);
     // ordered compaction: one wave at a time keeps lists sorted by s
     for (int w = 0; w < 4; ++w) {
       if ((int)(threadIdx.x >> 6) == w) {
@@ -619,7 +675,6 @@ __global__ __launch_bounds__(256) void ff_classify(ClassArgs a) {
         if (lane == 0) {
           nn += __popcll(mn);
           nb += __popcll(mb);
-          if (base + 64 * w < a.ncell) a.near_mask[(size_t)t * a.mask_words + (base >> 6) + w] = mn;
         }
       }
       __syncthreads();
@@ -629,6 +684,69 @@ __global__ __launch_bounds__(256) void ff_classify(ClassArgs a) {
     a.near_count[t] = nn;
     a.band_count[t] = nb;
   }
+}
+
+// Per target cell t: the far kernel's list of expansion records, in order.
+// Cells are visited in groups (4 siblings with the coarse level, else 1); a
+// group of 4 far cells whose parent p is far enough, angle >= r_par + rho_t +
+// rho_p, contributes the parent's record ncell + p, else every one of its
+// cells that is not near contributes its own.
+struct FarListArgs {
+  int ncell, group;  // group 4 (coarse level) or 1
+  const double* geo;
+  const double* pgeo;
+  double r_near, r_band, r_par;
+  int* far_list;   // [ncell][ncell]
+  int* far_count;
+};
+
+__global__ __launch_bounds__(256) void ff_far_list(FarListArgs a) {
+  __shared__ int sc[256];
+  __shared__ int base;
+  const int t = blockIdx.x;
+  const double* gt = a.geo + (size_t)t * 8;
+  if (threadIdx.x == 0) base = 0;
+  const int ng = a.ncell / a.group;
+  for (int g0 = 0; g0 < ng; g0 += 256) {
+    const int g = g0 + threadIdx.x;
+    int ent[4];
+    int n = 0;
+    if (g < ng) {
+      int cls[4];
+      bool all_far = true;
+      for (int k = 0; k < a.group; ++k) {
+        cls[k] = cell_class(gt, a.geo + (size_t)(g * a.group + k) * 8, a.r_near, a.r_band);
+        all_far = all_far && cls[k] == 0;
+      }
+      bool parent = false;
+      if (a.group == 4 && all_far) {
+        const double* gp = a.pgeo + (size_t)g * 4;
+        const double cosang = gt[4] * gp[0] + gt[5] * gp[1] + gt[6] * gp[2];
+        parent = acos(fmin(1.0, fmax(-1.0, cosang))) >= a.r_par + gt[7] + gp[3];
+      }
+      if (parent) {
+        ent[n++] = a.ncell + g;
+      } else {
+        for (int k = 0; k < a.group; ++k)
+          if (cls[k] != 2) ent[n++] = g * a.group + k;
+      }
+    }
+    // ordered placement: inclusive scan of the counts over the block
+    sc[threadIdx.x] = n;
+    __syncthreads();
+    for (int d = 1; d < 256; d <<= 1) {
+      const int v = threadIdx.x >= (unsigned)d ? sc[threadIdx.x - d] : 0;
+      __syncthreads();
+      sc[threadIdx.x] += v;
+      __syncthreads();
+    }
+    int* out = a.far_list + (size_t)t * a.ncell + base + sc[threadIdx.x] - n;
+    for (int i = 0; i < n; ++i) out[i] = ent[i];
+    __syncthreads();
+    if (threadIdx.x == 255) base += sc[255];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) a.far_count[t] = base;
 }
 
 // --------------------------------------------------------- near field
@@ -751,15 +869,16 @@ __global__ __launch_bounds__(256) void ff_near_kernel(NearArgs a) {
 }
 
 // ------------------------------------------------------- far (expansion)
-// Every cell that is not near a target's cell, by its expansion.  The cells
-// are split in gridDim.y contiguous ranges, one FP64 partial per range, so
-// that the launch has several rounds of waves and its last round is short.
+// Every record of a target cell's far list (cells not near it, or their
+// parents) by its expansion.  The list is split in gridDim.y contiguous
+// ranges, one FP64 partial per range, so that the launch has several rounds
+// of waves and its last round is short.
 struct FarArgs {
   const lpm::Vec4<float>* tgt;
   const int* tgt_start;
-  const float* cell;          // far records
-  const unsigned long long* near_mask;
-  int mask_words;
+  const float* cell;          // far records: cells, then parents
+  const int* far_list;        // [ncell][ncell] record indices per target cell
+  const int* far_count;
   const int2* items;
   const int* n_items;  // device count
   int ncell, nt;
@@ -793,21 +912,18 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7))) void f
   for (int r = 0; r < kR; ++r)
 #pragma unroll
     for (int c = 0; c < 3; ++c) S[r][c] = C[r][c] = 0.f;
-  const unsigned long long* near = a.near_mask + (size_t)t * a.mask_words;
-  const int c_lo = (int)((long long)a.ncell * blockIdx.y / gridDim.y);
-  const int c_hi = (int)((long long)a.ncell * (blockIdx.y + 1) / gridDim.y);
-  // one compensated block per 64-cell word of the mask; the far cells of a
-  // word are its clear bits inside [c_lo, c_hi), visited lowest first
-  for (int wd = c_lo >> 6; wd < (c_hi + 63) >> 6; ++wd) {
-    const int b_lo = max(c_lo - 64 * wd, 0), b_hi = min(c_hi - 64 * wd, 64);
-    const unsigned long long span = (b_hi == 64 ? ~0ull : (1ull << b_hi) - 1) & ~((1ull << b_lo) - 1);
-    unsigned long long far = ~near[wd] & span;
+  const int* list = a.far_list + (size_t)t * a.ncell;
+  const int cnt = a.far_count[t];
+  const int q_lo = (int)((long long)cnt * blockIdx.y / gridDim.y);
+  const int q_hi = (int)((long long)cnt * (blockIdx.y + 1) / gridDim.y);
+  // one compensated block per 64 records of this part of the list
+  for (int b0 = q_lo; b0 < q_hi; b0 += 64) {
+    const int b1 = min(q_hi, b0 + 64);
     float p[kR][3];
 #pragma unroll
     for (int r = 0; r < kR; ++r) p[r][0] = p[r][1] = p[r][2] = 0.f;
-    while (far) {
-      const int s = 64 * wd + __builtin_ctzll(far);
-      far &= far - 1;
+    for (int qi = b0; qi < b1; ++qi) {
+      const int s = list[qi];
       const float* cr = a.cell + (size_t)s * kCellRec;
       constexpr int NV = 4 + kMomOff[P] + (P + 1) * (P + 2) / 2;  // c, e_c, M'0..M'P
       float cv[NV];
@@ -1177,7 +1293,11 @@ struct lpm_bve_ff {
   int end_bit = 0;
   size_t sort_bytes = 0;
   DevBuf trec, srec, far_rec, mlp_rec, geo, cen64, mom64;
-  DevBuf near_mask, near_list, near_count, band_list, band_count;
+  DevBuf near_list, near_count, band_list, band_count;
+  DevBuf far_list, far_count;  // expansion records per target cell (cells or parents)
+  DevBuf pgeo;                 // coarse level: parent geometry [ncell / 4][4]
+  int npar = 0;                // parents in use (0: no coarse level)
+  double r_par = 0.0;          // a parent replaces its 4 cells beyond this radius
   DevBuf items, mlp_items;
   DevBuf V;
   DevBuf Vpart;        // FP64 partials [near_split + far_split][3][n]: near, then far
@@ -1268,10 +1388,17 @@ int ff_eval(lpm_bve_ff* f, int mode, const double* x, const double* y, const dou
               x, y, z, q, f->area.as<double>(), f->radius, f->far_rec.as<float>(), f->mlp_rec.as<float>(),
               f->adir.as<double>(), f->geo.as<double>(), f->cen64.as<double>(), f->mom64.as<double>()};
   hipLaunchKernelGGL(ff_cell_geometry, dim3(nc), b256, 0, ctx->stream, ga);
-  ClassArgs ca{nc, f->geo.as<double>(), f->r_near, f->r_band,
-               f->near_mask.as<unsigned long long>(), (nc + 63) / 64, f->near_list.as<int>(),
+  if (f->npar) {
+    ParentArgs pa2{f->src_perm.as<int>(), f->src_start.as<int>(), x, y, z, q, f->area.as<double>(), f->radius,
+                   f->far_rec.as<float>() + (size_t)nc * kCellRec, f->pgeo.as<double>()};
+    hipLaunchKernelGGL(ff_parent_geometry, dim3(f->npar), b256, 0, ctx->stream, pa2);
+  }
+  ClassArgs ca{nc, f->geo.as<double>(), f->r_near, f->r_band, f->near_list.as<int>(),
                f->near_count.as<int>(), f->band_list.as<int>(), f->band_count.as<int>()};
   hipLaunchKernelGGL(ff_classify, dim3(nc), b256, 0, ctx->stream, ca);
+  FarListArgs la{nc, f->npar ? 4 : 1, f->geo.as<double>(), f->pgeo.as<double>(), f->r_near, f->r_band, f->r_par,
+                 f->far_list.as<int>(), f->far_count.as<int>()};
+  hipLaunchKernelGGL(ff_far_list, dim3(nc), b256, 0, ctx->stream, la);
   NearArgs na{f->trec.as<lpm::Vec4<float>>(), f->tgt_perm.as<int>(), f->tgt_start.as<int>(), f->srec.as<float>(),
               f->src_start.as<int>(), f->near_list.as<int>(), f->near_count.as<int>(), f->items.as<int2>(),
               f->item_count.as<int>(), nc, f->Vpart.as<double>(), f->n, f->near_split};
@@ -1282,9 +1409,9 @@ int ff_eval(lpm_bve_ff* f, int mode, const double* x, const double* y, const dou
     if (e0 && e1) LPM_HIP(ctx, hipEventRecord(e0, ctx->stream));
   }
   hipLaunchKernelGGL(ff_near_kernel, dim3((f->n_items + 3) / 4, f->near_split), b256, 0, ctx->stream, na);
-  FarArgs fa{f->trec.as<lpm::Vec4<float>>(), f->tgt_start.as<int>(), f->far_rec.as<float>(),
-             f->near_mask.as<unsigned long long>(), (nc + 63) / 64, f->items.as<int2>(), f->item_count.as<int>(), nc,
-             f->n, f->Vpart.as<double>() + (size_t)f->near_split * 3 * f->n};
+  FarArgs fa{f->trec.as<lpm::Vec4<float>>(), f->tgt_start.as<int>(), f->far_rec.as<float>(), f->far_list.as<int>(),
+             f->far_count.as<int>(), f->items.as<int2>(), f->item_count.as<int>(), nc, f->n,
+             f->Vpart.as<double>() + (size_t)f->near_split * 3 * f->n};
   const dim3 gfar((f->n_items + 3) / 4, f->far_split);
   if (f->order == 2) hipLaunchKernelGGL(ff_far_kernel<2>, gfar, b256, 0, ctx->stream, fa);
   else if (f->order == 3) hipLaunchKernelGGL(ff_far_kernel<3>, gfar, b256, 0, ctx->stream, fa);
@@ -1424,12 +1551,13 @@ int ff_load(lpm_bve_ff* f, int n, const double* x, const double* y, const double
       chk(ff_upload(ctx, f->mlp_items, nullptr, (size_t)f->n_mlp_items * sizeof(int2))) &&
       chk(ff_upload(ctx, f->trec, nullptr, (size_t)n * 16)) &&
       chk(ff_upload(ctx, f->srec, nullptr, std::max<size_t>((size_t)f->ns, 1) * lpm::SREC * 4)) &&
-      chk(ff_upload(ctx, f->far_rec, nullptr, (size_t)ncell * kCellRec * 4)) &&
+      chk(ff_upload(ctx, f->far_rec, nullptr, (size_t)(ncell + ncell / 4) * kCellRec * 4)) &&
       chk(ff_upload(ctx, f->mlp_rec, nullptr, (size_t)ncell * kMlpRec * 4)) &&
       chk(ff_upload(ctx, f->geo, nullptr, (size_t)ncell * 8 * 8)) &&
       chk(ff_upload(ctx, f->cen64, nullptr, (size_t)ncell * 3 * 8)) &&
       chk(ff_upload(ctx, f->mom64, nullptr, (size_t)ncell * kNMom * 8)) &&
-      chk(ff_upload(ctx, f->near_mask, nullptr, (size_t)ncell * ((ncell + 63) / 64) * 8)) &&
+      chk(ff_upload(ctx, f->far_list, nullptr, (size_t)ncell * ncell * 4)) &&
+      chk(ff_upload(ctx, f->far_count, nullptr, (size_t)ncell * 4)) &&
       chk(ff_upload(ctx, f->near_list, nullptr, (size_t)ncell * ncell * 4)) &&
       chk(ff_upload(ctx, f->near_count, nullptr, (size_t)ncell * 4)) &&
       chk(ff_upload(ctx, f->band_list, nullptr, (size_t)ncell * ncell * 4)) &&
@@ -1812,12 +1940,37 @@ int lpm_bve_ff_cells(lpm_bve_ff* f, int* near_cells, int* band_cells) {
   return LPM_OK;
 }
 
+int lpm_bve_ff_set_parents(lpm_bve_ff* f, double parent_radius) {
+  if (!f) return LPM_ERR_ARG;
+  lpm_ctx* ctx = f->ctx;
+  ctx->err.clear();
+  if (!(parent_radius >= 0.0)) return fail(ctx, LPM_ERR_ARG, "lpm_bve_ff_set_parents: parent_radius must be >= 0");
+  if (parent_radius > 0.0 && f->ncell % 4 != 0)
+    return fail(ctx, LPM_ERR_ARG, "lpm_bve_ff_set_parents: the coarse level needs a cell count divisible by 4");
+  LPM_HIP(ctx, hipSetDevice(ctx->device));
+  if (parent_radius > 0.0) LPM_HIP(ctx, f->pgeo.ensure((size_t)(f->ncell / 4) * 4 * sizeof(double)));
+  f->r_par = parent_radius;
+  f->npar = parent_radius > 0.0 ? f->ncell / 4 : 0;
+  return LPM_OK;
+}
+
+int lpm_bve_ff_far_counts(lpm_bve_ff* f, int* far_records) {
+  if (!f || !far_records) return LPM_ERR_ARG;
+  lpm_ctx* ctx = f->ctx;
+  ctx->err.clear();
+  if (int rc = ff_usable(f)) return rc;
+  LPM_HIP(ctx, hipSetDevice(ctx->device));
+  LPM_HIP(ctx, hipMemcpyAsync(far_records, f->far_count.p, (size_t)f->ncell * 4, hipMemcpyDeviceToHost, ctx->stream));
+  LPM_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return LPM_OK;
+}
+
 int lpm_bve_ff_delete(lpm_bve_ff* f) {
   if (!f) return LPM_ERR_ARG;
   (void)hipStreamSynchronize(f->ctx->stream);
   DevBuf* all[] = {&f->x, &f->y, &f->z, &f->q, &f->u, &f->v, &f->w, &f->area, &f->xi, &f->yi, &f->zi, &f->qi,
                    &f->pu, &f->pv, &f->pw, &f->src_perm, &f->src_start, &f->tgt_perm, &f->tgt_start, &f->tgt_cell,
-                   &f->trec, &f->srec, &f->far_rec, &f->mlp_rec, &f->geo, &f->cen64, &f->mom64, &f->near_mask,
+                   &f->trec, &f->srec, &f->far_rec, &f->mlp_rec, &f->geo, &f->cen64, &f->mom64, &f->far_list, &f->far_count, &f->pgeo,
                    &f->near_list, &f->near_count, &f->band_list, &f->band_count, &f->items, &f->mlp_items, &f->V,
                    &f->list_plain, &f->tgt_plain, &f->src_plain, &f->partial_plain, &f->ptrs, &f->err_part, &f->sx,
                    &f->sy, &f->sid, &f->src_key, &f->key_t, &f->key_s, &f->iota, &f->cellof, &f->active,

@@ -572,6 +572,20 @@ module lpm_direct_f
       integer(c_int) :: near_cells(*), band_cells(*)
     end function
 
+    ! coarse level of the expansion part (0 turns it off) and the expansion
+    ! records per target cell of the last evaluation (length ncell)
+    integer(c_int) function lpm_bve_ff_set_parents(ff, parent_radius) bind(C, name="lpm_bve_ff_set_parents")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ff
+      real(c_double), value :: parent_radius
+    end function
+
+    integer(c_int) function lpm_bve_ff_far_counts(ff, far_records) bind(C, name="lpm_bve_ff_far_counts")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ff
+      integer(c_int) :: far_records(*)
+    end function
+
     integer(c_int) function lpm_bve_ff_samples(ff, offset, count, inputs, targets, ids) &
         bind(C, name="lpm_bve_ff_samples")
       import :: c_int, c_ptr, c_int64_t, c_float, c_int32_t

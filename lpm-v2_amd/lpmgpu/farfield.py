@@ -28,6 +28,8 @@ L.extra_signatures({
     "lpm_bve_ff_set_state": ([_vp, L._pd, L._pd, L._pd, L._pd], _i),
     "lpm_bve_ff_set_particles": ([_vp, _i, L._pd, L._pd, L._pd, L._pd, L._pd, L._pi], _i),
     "lpm_bve_ff_cells": ([_vp, L._pi, L._pi], _i),
+    "lpm_bve_ff_set_parents": ([_vp, _d], _i),
+    "lpm_bve_ff_far_counts": ([_vp, L._pi], _i),
     "lpm_bve_ff_particle_cells": ([_vp, L._pi], _i),
     "lpm_bve_ff_delete": ([_vp], _i),
 })
@@ -44,7 +46,7 @@ class FarFieldBVE:
     training samples taking the nearest of 4 band cells."""
 
     def __init__(self, ctx, mesh, relvort, rotation_rate=0.0, near_radius=0.1, band_radius=0.2, radius=1.0,
-                 order=4, sample_draws=4):
+                 order=4, sample_draws=4, parent_radius=0.0):
         if mesh.face_cell is None or mesh.cell_level < 0:
             raise ValueError("mesh needs coarse cells (cell_level >= 0)")
         self.ctx = ctx
@@ -60,6 +62,20 @@ class FarFieldBVE:
                                           order | ((sample_draws & 7) << 3)))
         self.handle = h
         ctx.adopt(self)
+        if parent_radius > 0.0:
+            self.set_parents(parent_radius)
+
+    def set_parents(self, parent_radius):
+        """Coarse level of the expansion part (lpm_bve_ff_set_parents): beyond
+        `parent_radius` four sibling far cells are one expansion; 0 turns it off.
+        Needs the cells of a refined mesh (cells 4p .. 4p+3 share parent p)."""
+        self.ctx.check(self.ctx._lib.lpm_bve_ff_set_parents(self.handle, float(parent_radius)))
+
+    def far_counts(self):
+        """Expansion records per target cell in the last evaluation."""
+        a = np.zeros(self.ncell, np.int32)
+        self.ctx.check(self.ctx._lib.lpm_bve_ff_far_counts(self.handle, pi(a)))
+        return a
 
     @staticmethod
     def particle_cells(mesh):

@@ -171,3 +171,120 @@ def test_network_gain_and_judge(ctx):
     st = f.state()
     assert np.abs(np.sqrt(st["x"] ** 2 + st["y"] ** 2 + st["z"] ** 2) - 1).max() < 1e-14
     f.delete()
+
+
+def _geometry(P, area, cells, act, ncell):
+    """Per cell, as the GPU computes it in FP64: direction and angular radius of
+    the area-weighted source centroid, and of the (unweighted) target sum."""
+    src = []
+    for s in range(ncell):
+        idx = np.flatnonzero(act & (cells == s))
+        c = (P[idx] * area[idx, None]).sum(0) / area[idx].sum()
+        u = c / np.linalg.norm(c)
+        src.append((idx, c, u, np.arccos(np.clip(P[idx] @ u, -1, 1)).max()))
+    tgt = []
+    for t in range(ncell):
+        idx = np.flatnonzero(cells == t)
+        v = P[idx].sum(0)
+        v /= np.linalg.norm(v)
+        tgt.append((v, np.arccos(np.clip(P[idx] @ v, -1, 1)).max()))
+    return src, tgt
+
+
+def test_coarse_level_matches_numpy(ctx):
+    """lpm_bve_ff_set_parents: beyond the parent radius four sibling far cells
+    are one expansion of their union about its own centroid.  Restated in numpy
+    for 40 targets: the near cells summed directly (self excluded), every far
+    record (a cell, or a parent where its four cells are far and the parent
+    criterion holds) by its Taylor expansion from Cauchy's integral.  A parent
+    radius no pair reaches gives the plain cell list, bitwise."""
+    m, zeta = _case(5, 3)
+    near, band, rpar, order = 0.1, 0.2, 0.3, 4
+    f = FarFieldBVE(ctx, m, zeta, near_radius=near, band_radius=band, order=order)
+    e_cells = np.array(f.velocity(EXPANSION))
+    n_cells = f.far_counts()
+    f.set_parents(10.0)
+    np.testing.assert_array_equal(np.array(f.velocity(EXPANSION)), e_cells)
+    np.testing.assert_array_equal(f.far_counts(), n_cells)
+    f.set_parents(rpar)
+    got = np.array(f.velocity(EXPANSION))
+    n_par = f.far_counts()
+    cells = f.binning()
+    plain = np.array(f.velocity(PLAIN))
+    print(f"records per target cell: {n_cells.mean():.1f} cells only, {n_par.mean():.1f} with parents; error vs plain "
+          f"{rel_err_max(e_cells, plain):.3e} / {rel_err_max(got, plain):.3e}", flush=True)
+    assert n_par.mean() < 0.6 * n_cells.mean()
+    assert rel_err_max(got, plain) < 2.0 * rel_err_max(e_cells, plain)
+
+    ncell = f.ncell
+    P, act, w = m.xyz(), m.active == 1, zeta * m.area
+    src, tgt = _geometry(P, m.area, cells, act, ncell)
+    par = []
+    for g in range(ncell // 4):
+        idx = np.concatenate([src[4 * g + k][0] for k in range(4)])
+        c = (P[idx] * m.area[idx, None]).sum(0) / m.area[idx].sum()
+        u = c / np.linalg.norm(c)
+        par.append((idx, c, u, np.arccos(np.clip(P[idx] @ u, -1, 1)).max()))
+
+    def ang(a, b):
+        return np.arccos(np.clip(a @ b, -1, 1))
+
+    scale = np.sqrt((plain ** 2).sum(0)).max()
+    rng = np.random.default_rng(7)
+    checked, n_parent_records = 0, 0
+    for p in rng.choice(m.n, 60, replace=False):
+        t = cells[p]
+        v, rt = tgt[t]
+        margin = np.inf
+        V = np.zeros(3)
+        x = P[p]
+        records = 0
+        for g in range(ncell // 4):
+            cls = []
+            for s in range(4 * g, 4 * g + 4):
+                a = ang(v, src[s][2])
+                pad = rt + src[s][3]
+                margin = min(margin, abs(a - near - pad), abs(a - band - pad))
+                cls.append(2 if a < near + pad else (1 if a < band + pad else 0))
+            ap = ang(v, par[g][2])
+            if all(k == 0 for k in cls):
+                margin = min(margin, abs(ap - rpar - rt - par[g][3]))
+            if all(k == 0 for k in cls) and ap >= rpar + rt + par[g][3]:
+                idx, c = par[g][0], par[g][1]
+                V += taylor_cell_sum(w[idx], P[idx], c, x, order)
+                records += 1
+                n_parent_records += 1
+                continue
+            for k, s in enumerate(range(4 * g, 4 * g + 4)):
+                idx, c = src[s][0], src[s][1]
+                if cls[k] == 2:
+                    idx = idx[idx != p]
+                    V += (w[idx, None] * P[idx] / (1.0 - P[idx] @ x)[:, None]).sum(0)
+                else:
+                    V += taylor_cell_sum(w[idx], P[idx], c, x, order)
+                    records += 1
+        if margin < 1e-9:  # a class decision within rounding of its threshold
+            continue
+        assert records == n_par[t]
+        want = np.cross(V, x) / (4 * np.pi)
+        assert np.abs(got[:, p] - want).max() < 2e-5 * scale, (p, got[:, p], want)
+        checked += 1
+        if checked == 40:
+            break
+    assert checked == 40 and n_parent_records > 0
+    with pytest.raises(Exception):
+        f.set_parents(-1.0)
+    f.delete()
+
+
+def test_coarse_level_needs_sibling_cells(ctx):
+    """A cell count not divisible by 4 has no sibling groups: the coarse level
+    is refused."""
+    m = mesh.cubed_sphere_mesh(3, cell_level=0)
+    zeta = ic.rh4_vorticity(m.x, m.y, m.z)
+    f = FarFieldBVE(ctx, m, zeta, near_radius=0.1, band_radius=0.2)
+    assert f.ncell == 6
+    with pytest.raises(Exception):
+        f.set_parents(0.6)
+    f.set_parents(0.0)
+    f.delete()
