@@ -68,7 +68,7 @@ constexpr int kMlpRec = 32;  // floats per MLP cell record
 constexpr int kMlpNct = 2;   // 16-target column tiles per network pass
 constexpr int kMlpTile = 16 * kMlpNct;  // targets per network work item
 constexpr int kLocR = 2;  // particles per lane in the binning kernel
-// The near and far kernels split their work (near lists, far cell ranges) in
+// The near and far kernels split their work (near lists, far record lists) in
 // up to kSplitMax parts, enough for about kSplitRounds rounds of resident waves.
 constexpr double kSplitRounds = 6.0;
 constexpr int kSplitMax = 8;
