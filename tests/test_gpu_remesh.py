@@ -138,12 +138,13 @@ def test_reload_errors_leave_the_solver_usable(ctx):
     s.delete()
 
 
-def test_far_field_reload_keeps_cells_and_network(ctx):
-    """The far field keeps its cells and network across a reload to a finer
-    set; its plain velocity is the direct sum of the new set and the
-    expansion stays within its usual error."""
+@pytest.mark.parametrize("parent", [0.0, 0.9])
+def test_far_field_reload_keeps_cells_and_network(ctx, parent):
+    """The far field keeps its cells, its coarse level and its network across a
+    reload to a finer set; its plain velocity is the direct sum of the new set
+    and the expansion stays within its usual error."""
     m4 = mesh.icosahedral_mesh(4, cell_level=2)
-    f = FarFieldBVE(ctx, m4, _vort(m4), near_radius=0.3, band_radius=0.6)
+    f = FarFieldBVE(ctx, m4, _vort(m4), near_radius=0.3, band_radius=0.6, parent_radius=parent)
     p0 = f.params()
     m5 = mesh.icosahedral_mesh(5, cell_level=2)
     zeta = _vort(m5)
@@ -154,8 +155,12 @@ def test_far_field_reload_keeps_cells_and_network(ctx):
     ref = ctx.bve_sphere_velocity(m5.x, m5.y, m5.z, zeta, m5.area, m5.active, precision="fp64")
     e_plain = rel_err_max(plain, ref)
     e_exp = rel_err_max(f.velocity(EXPANSION), plain)
-    fresh = FarFieldBVE(ctx, m5, zeta, near_radius=0.3, band_radius=0.6)
+    fresh = FarFieldBVE(ctx, m5, zeta, near_radius=0.3, band_radius=0.6, parent_radius=parent)
     e_fresh = rel_err_max(fresh.velocity(EXPANSION), fresh.velocity(PLAIN))
+    if parent > 0:  # the coarse level is in use after the reload: fewer records than non-near cells
+        for g in (f, fresh):
+            near, _ = g.cells()
+            assert (g.far_counts() + near).mean() < g.ncell - 1
     print("ff reload: plain vs fp64", e_plain, "expansion", e_exp, "fresh object", e_fresh)
     assert e_plain < 1e-6
     assert e_exp < 3 * e_fresh + 1e-7
