@@ -635,6 +635,10 @@ def main():
             cpu["cpu_model"] = port.pop("cpu_model")
             cpu["host"] = port.pop("host")
             cpu["port"] = port
+            one = reference_cpu_baseline(m, zeta, min(3.0, args.cpu_seconds / 4), 1)
+            if one is not None:
+                cpu["one_core"] = {"value": one["value"], "sample": one["sample"]}
+            log(f"[bench] CPU baseline (reference, one rank) {one}")
 
     out = {
         "metric": METRIC,
