@@ -191,7 +191,8 @@ def _geometry(P, area, cells, act, ncell):
     return src, tgt
 
 
-def test_coarse_level_matches_numpy(ctx):
+@pytest.mark.parametrize("order", [2, 3, 4])
+def test_coarse_level_matches_numpy(ctx, order):
     """lpm_bve_ff_set_parents: beyond the parent radius four sibling far cells
     are one expansion of their union about its own centroid.  Restated in numpy
     for 40 targets: the near cells summed directly (self excluded), every far
@@ -199,7 +200,7 @@ def test_coarse_level_matches_numpy(ctx):
     criterion holds) by its Taylor expansion from Cauchy's integral.  A parent
     radius no pair reaches gives the plain cell list, bitwise."""
     m, zeta = _case(5, 3)
-    near, band, rpar, order = 0.1, 0.2, 0.3, 4
+    near, band, rpar = 0.1, 0.2, 0.3
     f = FarFieldBVE(ctx, m, zeta, near_radius=near, band_radius=band, order=order)
     e_cells = np.array(f.velocity(EXPANSION))
     n_cells = f.far_counts()
@@ -214,7 +215,7 @@ def test_coarse_level_matches_numpy(ctx):
     print(f"records per target cell: {n_cells.mean():.1f} cells only, {n_par.mean():.1f} with parents; error vs plain "
           f"{rel_err_max(e_cells, plain):.3e} / {rel_err_max(got, plain):.3e}", flush=True)
     assert n_par.mean() < 0.6 * n_cells.mean()
-    assert rel_err_max(got, plain) < 2.0 * rel_err_max(e_cells, plain)
+    assert rel_err_max(got, plain) < 3.0 * rel_err_max(e_cells, plain)
 
     ncell = f.ncell
     P, act, w = m.xyz(), m.active == 1, zeta * m.area
