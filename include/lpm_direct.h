@@ -440,6 +440,10 @@ int lpm_bve_ff_train(lpm_bve_ff *ff, int epochs, int batch, double lr, uint64_t 
  * mode, replace the collected set by n_samples samples of the judged state and
  * train `epochs` epochs on it.  n_samples = 0 (the default) turns it off. */
 int lpm_bve_ff_set_online(lpm_bve_ff *ff, int n_samples, int epochs, int batch, double lr);
+/* Fine-tune only at judges whose error is above fraction * tol (fraction in
+ * [0, 1]; 0, the default, fine-tunes at every judge that keeps the learned
+ * mode). */
+int lpm_bve_ff_set_online_from(lpm_bve_ff *ff, double fraction);
 /* Copy samples [offset, offset + count): in [count][13], target [count][3] and,
  * when ids is not NULL, ids [count][2] = (target particle, source cell). */
 int lpm_bve_ff_samples(lpm_bve_ff *ff, int64_t offset, int64_t count, float *in, float *target,

@@ -1313,6 +1313,7 @@ struct lpm_bve_ff {
   bool trained = false;
   double ys = 0.0;  // RMS of the training targets (0: not fixed yet)
   int online_n = 0, online_epochs = 1, online_batch = 4096, online_rounds = 0;
+  double online_from = 0.0;  // fine-tune only at judged errors above online_from * tol
   double online_lr = 2e-4;
   bool loaded = false;  // false after a failed particle reload
 };
@@ -1790,6 +1791,12 @@ int lpm_bve_ff_set_online(lpm_bve_ff* f, int n_samples, int epochs, int batch, d
   return LPM_OK;
 }
 
+int lpm_bve_ff_set_online_from(lpm_bve_ff* f, double fraction) {
+  if (!f || !(fraction >= 0.0 && fraction <= 1.0)) return LPM_ERR_ARG;
+  f->online_from = fraction;
+  return LPM_OK;
+}
+
 int lpm_bve_ff_set_params(lpm_bve_ff* f, const float* params, int64_t count) {
   if (!f || !params) return LPM_ERR_ARG;
   lpm_ctx* ctx = f->ctx;
@@ -1851,7 +1858,7 @@ int lpm_bve_ff_steps(lpm_bve_ff* f, double dt, int nsteps, int mode, int judge_e
         LPM_HIP(ctx, hipMemcpyAsync(f->u.p, f->pu.p, (size_t)f->n * 8, hipMemcpyDeviceToDevice, ctx->stream));
         LPM_HIP(ctx, hipMemcpyAsync(f->v.p, f->pv.p, (size_t)f->n * 8, hipMemcpyDeviceToDevice, ctx->stream));
         LPM_HIP(ctx, hipMemcpyAsync(f->w.p, f->pw.p, (size_t)f->n * 8, hipMemcpyDeviceToDevice, ctx->stream));
-      } else if (cur == 2 && f->online_n > 0) {
+      } else if (cur == 2 && f->online_n > 0 && err > f->online_from * tol) {
         // fine-tune on direct sums of the judged state for the steps that follow
         f->n_collected = 0;
         if (int rc = lpm_bve_ff_collect(f, f->online_n, 0x5eedull + (uint64_t)(++f->online_rounds), nullptr))

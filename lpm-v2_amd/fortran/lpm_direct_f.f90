@@ -455,6 +455,13 @@ module lpm_direct_f
       real(c_double), value :: lr
     end function
 
+    ! fine-tune only at judged errors above fraction * tol (0: at every judge)
+    integer(c_int) function lpm_bve_ff_set_online_from(ff, fraction) bind(C, name="lpm_bve_ff_set_online_from")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ff
+      real(c_double), value :: fraction
+    end function
+
     integer(c_int) function lpm_bve_ff_steps(ff, dt, nsteps, mode, judge_every, tol, mode_out, last_error, &
         judged) bind(C, name="lpm_bve_ff_steps")
       import :: c_int, c_ptr, c_double

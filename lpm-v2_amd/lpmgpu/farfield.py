@@ -20,6 +20,7 @@ L.extra_signatures({
     "lpm_bve_ff_collect": ([_vp, _i, ctypes.c_uint64, ctypes.POINTER(ctypes.c_int64)], _i),
     "lpm_bve_ff_train": ([_vp, _i, _i, _d, ctypes.c_uint64, L._pd], _i),
     "lpm_bve_ff_set_online": ([_vp, _i, _i, _i, _d], _i),
+    "lpm_bve_ff_set_online_from": ([_vp, _d], _i),
     "lpm_bve_ff_samples": ([_vp, ctypes.c_int64, ctypes.c_int64, _pf, _pf, L._pi], _i),
     "lpm_bve_ff_set_params": ([_vp, _pf, ctypes.c_int64], _i),
     "lpm_bve_ff_get_params": ([_vp, _pf, ctypes.c_int64], _i),
@@ -112,10 +113,12 @@ class FarFieldBVE:
         self.ctx.check(self.ctx._lib.lpm_bve_ff_train(self.handle, epochs, batch, lr, seed, ctypes.byref(loss)))
         return loss.value
 
-    def set_online(self, n_samples, epochs=1, batch=4096, lr=2e-4):
+    def set_online(self, n_samples, epochs=1, batch=4096, lr=2e-4, above=0.0):
         """Fine-tune on n_samples direct-sum samples at every judge that keeps
-        the learned mode (0 turns it off)."""
+        the learned mode (0 turns it off) and whose error is above `above` x
+        tol (lpm_bve_ff_set_online_from; 0: at every such judge)."""
         self.ctx.check(self.ctx._lib.lpm_bve_ff_set_online(self.handle, n_samples, epochs, batch, lr))
+        self.ctx.check(self.ctx._lib.lpm_bve_ff_set_online_from(self.handle, float(above)))
 
     def params(self):
         p = np.zeros(N_PARAMS, np.float32)

@@ -289,3 +289,23 @@ def test_coarse_level_needs_sibling_cells(ctx):
         f.set_parents(0.6)
     f.set_parents(0.0)
     f.delete()
+
+
+def test_online_fine_tuning_threshold(ctx):
+    """lpm_bve_ff_set_online_from: a judge fine-tunes the network only when its
+    error is above the given fraction of tol."""
+    m, zeta = _case(4, 2)
+    f = FarFieldBVE(ctx, m, zeta, rotation_rate=2 * np.pi, near_radius=0.1, band_radius=0.45)
+    f.collect(20000, seed=1)
+    f.train(epochs=2, batch=1024, lr=1e-3, seed=2)
+    f.set_online(2000, epochs=1, batch=512, lr=1e-3, above=1.0)  # errors never exceed tol
+    p0 = f.params()
+    r = f.steps(0.01, 2, LEARNED, judge_every=1, tol=1.0)
+    assert r["mode"] == LEARNED and r["judged"] == 2
+    np.testing.assert_array_equal(f.params(), p0)
+    f.set_online(2000, epochs=1, batch=512, lr=1e-3, above=0.0)
+    f.steps(0.01, 1, LEARNED, judge_every=1, tol=1.0)
+    assert not np.array_equal(f.params(), p0)
+    with pytest.raises(Exception):
+        f.set_online(2000, above=1.5)
+    f.delete()
